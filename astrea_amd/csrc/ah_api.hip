@@ -1,0 +1,433 @@
+// C ABI of libastrea_hip.so (declared in include/astrea_hip.h): plans,
+// workspace layout, the Runge-Kutta stage programs and the launch sequence of
+// one time step.  No HIP allocation happens after ah_plan_create.
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "../../include/astrea_hip.h"
+#include "ah_control.h"
+#include "ah_launch.h"
+
+using namespace ah;
+
+namespace {
+
+struct Stage {
+    int in;  // register holding the stage input (0 = state)
+    int op;  // RkOp applied after the stage's L
+};
+
+// evolvers.py:79-206, one entry per evolve_space call of the reference
+std::vector<Stage> rk_program(int rk) {
+    switch (rk) {
+        case AH_EULER: return {{0, OP_EULER}};
+        case AH_SSPRK22: return {{0, OP_22_0}, {1, OP_22_1}};
+        case AH_SSPRK33: return {{0, OP_33_0}, {1, OP_33_1}, {2, OP_33_2}};
+        case AH_SSPRK43: return {{0, OP_43_0}, {1, OP_43_1}, {2, OP_43_2}, {3, OP_43_3}};
+        case AH_RK4: return {{0, OP_RK4_0}, {1, OP_RK4_1}, {2, OP_RK4_2}, {3, OP_RK4_3}};
+        case AH_SSPRK53: return {{0, OP_53_0}, {1, OP_53_1}, {2, OP_53_2}, {3, OP_53_3}, {4, OP_53_4}};
+        case AH_SSPRK54: return {{0, OP_54_0}, {1, OP_54_1}, {2, OP_54_2}, {3, OP_54_3}, {4, OP_54_4}};
+        case AH_SSPRK104: {
+            // the reference evaluates evolve_space 11 times per step (evolvers.py:87-103)
+            std::vector<Stage> p = {{0, OP_104_FIRST}};
+            for (int i = 0; i < 4; ++i) p.push_back({1, OP_104_ACC1});
+            p.push_back({1, OP_104_K5});
+            p.push_back({3, OP_104_K2FIRST});
+            for (int i = 0; i < 3; ++i) p.push_back({2, OP_104_ACC2});
+            p.push_back({2, OP_104_FINAL});
+            return p;
+        }
+    }
+    return {};
+}
+
+int popcount8(int m) {
+    int n = 0;
+    for (int c = 0; c < 8; ++c) n += (m >> c) & 1;
+    return n;
+}
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        if ((x) != hipSuccess) return AH_ERR_HIP;  \
+    } while (0)
+
+}  // namespace
+
+struct ah_plan {
+    ah_scheme sch;
+    int nprob = 0;
+    std::vector<ah_problem> probs;
+    int nv = 0;
+    int varidx[8];
+    long long total = 0;
+    std::vector<long long> offs;
+    KernelSet ks;
+    std::vector<Stage> prog;
+    int nregs = 0;  // K registers + saved L registers used by the program
+    // device tables
+    ProbDesc* d_pd = nullptr;
+    ProbState* d_ps = nullptr;
+    int* d_cell_prob = nullptr;
+    Tile* d_t1 = nullptr;
+    Tile* d_tr[2] = {nullptr, nullptr};
+    Tile* d_tf[2] = {nullptr, nullptr};
+    int n1t = 0, nrt[2] = {0, 0}, nft[2] = {0, 0};
+    // bound memory
+    double* U = nullptr;
+    double* regs[9] = {nullptr};
+    double* Lbuf = nullptr;
+    double* Dbuf = nullptr;
+    void* wL = nullptr;
+    void* wR = nullptr;
+    void* bn = nullptr;
+    bool bound = false;
+};
+
+extern "C" {
+
+int ah_version(int* major, int* minor) {
+    if (major) *major = AH_VERSION_MAJOR;
+    if (minor) *minor = AH_VERSION_MINOR;
+    return AH_OK;
+}
+
+const char* ah_status_str(int status) {
+    switch (status) {
+        case AH_OK: return "ok";
+        case AH_ERR_INVALID: return "invalid argument or option combination";
+        case AH_ERR_UNSUPPORTED: return "option combination not implemented on the device path";
+        case AH_ERR_HIP: return "HIP runtime error";
+        case AH_ERR_NOMEM: return "workspace too small or allocation failed";
+        case AH_ERR_STATE: return "plan not bound or used out of order";
+    }
+    return "unknown status";
+}
+
+int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_plan** out) {
+    if (!sch || !probs || !out || nprob <= 0) return AH_ERR_INVALID;
+    if (sch->dims != 1 && sch->dims != 2) return AH_ERR_INVALID;
+    if (sch->magnetic_2d) return AH_ERR_UNSUPPORTED;
+    ah_plan* p = new (std::nothrow) ah_plan();
+    if (!p) return AH_ERR_NOMEM;
+    p->sch = *sch;
+    p->nprob = nprob;
+    p->probs.assign(probs, probs + nprob);
+    p->nv = popcount8(sch->varmask);
+    for (int c = 0, k = 0; c < 8; ++c)
+        if ((sch->varmask >> c) & 1) p->varidx[k++] = c;
+    bool ok;
+    if (sch->precision == AH_FP64)
+        ok = sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks)
+                            : kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks);
+    else
+        ok = sch->dims == 1 ? kernels_1d_fp32(sch->varmask, sch->recon, sch->solver, &p->ks)
+                            : kernels_2d_fp32(sch->varmask, sch->recon, sch->solver, &p->ks);
+    p->prog = rk_program(sch->rk);
+    if (!ok || p->prog.empty()) {
+        delete p;
+        return AH_ERR_UNSUPPORTED;
+    }
+    // registers: K1..K4 (1-4), saved L (5-8)
+    switch (sch->rk) {
+        case AH_EULER: p->nregs = 0; break;
+        case AH_SSPRK22: p->nregs = 1; break;
+        case AH_SSPRK33: p->nregs = 2; break;
+        case AH_SSPRK43: p->nregs = 3; break;
+        case AH_RK4: p->nregs = 7; break;
+        case AH_SSPRK53: p->nregs = 6; break;
+        case AH_SSPRK54: p->nregs = 5; break;
+        case AH_SSPRK104: p->nregs = 3; break;
+    }
+    // problem tables and tiles
+    std::vector<ProbDesc> pd(nprob);
+    long long total = 0;
+    std::vector<Tile> t1, tr[2], tf[2];
+    std::vector<int> cell_prob;
+    for (int i = 0; i < nprob; ++i) {
+        const ah_problem& q = probs[i];
+        const int n1 = sch->dims == 1 ? 1 : q.n1;
+        if (q.n0 <= 0 || n1 <= 0 || q.dx <= 0 || q.checkpoints <= 0 || (q.boundary != AH_EDGE && q.boundary != AH_WRAP)) {
+            delete p;
+            return AH_ERR_INVALID;
+        }
+        pd[i].n0 = q.n0;
+        pd[i].n1 = n1;
+        pd[i].bc = q.boundary;
+        pd[i].nprob_pad = 0;
+        pd[i].off = total;
+        pd[i].dx = q.dx;
+        pd[i].gamma = q.gamma;
+        pd[i].cfl = q.cfl;
+        pd[i].t_end = q.t_end;
+        pd[i].chkpt = q.t_end / q.checkpoints;
+        pd[i].max_steps = q.max_steps;
+        p->offs.push_back(total);
+        total += (long long)q.n0 * n1;
+        if (sch->dims == 1) {
+            for (int s0 = 0; s0 < q.n0; s0 += p->ks.ts1) t1.push_back({i, s0, 0, 0});
+        } else {
+            for (int ax = 0; ax < 2; ++ax) {
+                const int ns = ax == 0 ? q.n0 : n1, nt = ax == 0 ? n1 : q.n0;
+                for (int t0 = 0; t0 < nt; t0 += p->ks.rtt[ax])
+                    for (int s0 = 0; s0 < ns; s0 += p->ks.rts[ax]) tr[ax].push_back({i, s0, t0, 0});
+                for (int t0 = 0; t0 < nt; t0 += p->ks.ftt[ax])
+                    for (int s0 = 0; s0 < ns; s0 += p->ks.fts[ax]) tf[ax].push_back({i, s0, t0, 0});
+            }
+        }
+    }
+    p->total = total;
+    if (nprob > 1) {
+        cell_prob.resize(total);
+        for (int i = 0; i < nprob; ++i) std::fill(cell_prob.begin() + pd[i].off, cell_prob.begin() + pd[i].off +
+                                                                                     (long long)pd[i].n0 * pd[i].n1, i);
+    }
+    auto upload = [](auto** dst, const auto& v) -> bool {
+        if (v.empty()) return true;
+        if (hipMalloc((void**)dst, v.size() * sizeof(v[0])) != hipSuccess) return false;
+        return hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool up = upload(&p->d_pd, pd) && upload(&p->d_t1, t1) && upload(&p->d_tr[0], tr[0]) && upload(&p->d_tr[1], tr[1]) &&
+              upload(&p->d_tf[0], tf[0]) && upload(&p->d_tf[1], tf[1]) && upload(&p->d_cell_prob, cell_prob);
+    up = up && hipMalloc((void**)&p->d_ps, nprob * sizeof(ProbState)) == hipSuccess &&
+         hipMemset(p->d_ps, 0, nprob * sizeof(ProbState)) == hipSuccess;
+    p->n1t = (int)t1.size();
+    for (int ax = 0; ax < 2; ++ax) {
+        p->nrt[ax] = (int)tr[ax].size();
+        p->nft[ax] = (int)tf[ax].size();
+    }
+    if (!up) {
+        ah_plan_destroy(p);
+        return AH_ERR_HIP;
+    }
+    *out = p;
+    return AH_OK;
+}
+
+int ah_plan_destroy(ah_plan* p) {
+    if (!p) return AH_ERR_INVALID;
+    void* bufs[] = {p->d_pd, p->d_ps, p->d_cell_prob, p->d_t1, p->d_tr[0], p->d_tr[1], p->d_tf[0], p->d_tf[1]};
+    int rc = AH_OK;
+    for (void* b : bufs)
+        if (b && hipFree(b) != hipSuccess) rc = AH_ERR_HIP;
+    delete p;
+    return rc;
+}
+
+int ah_plan_layout(const ah_plan* p, long long* total_cells, int* nvars, int* varidx, long long* offsets) {
+    if (!p) return AH_ERR_INVALID;
+    if (total_cells) *total_cells = p->total;
+    if (nvars) *nvars = p->nv;
+    if (varidx)
+        for (int v = 0; v < p->nv; ++v) varidx[v] = p->varidx[v];
+    if (offsets)
+        for (int i = 0; i < p->nprob; ++i) offsets[i] = p->offs[i];
+    return AH_OK;
+}
+
+static size_t plane_bytes(const ah_plan* p, size_t elem) { return ((size_t)p->nv * p->total * elem + 255) & ~size_t(255); }
+
+int ah_plan_workspace_size(const ah_plan* p, size_t* bytes) {
+    if (!p || !bytes) return AH_ERR_INVALID;
+    size_t b = plane_bytes(p, 8) * (p->nregs + 1);  // registers + L
+    if (p->sch.dims == 2) {
+        b += plane_bytes(p, 8);                        // D
+        b += 2 * plane_bytes(p, p->ks.real_bytes);     // wL, wR
+        b += ((size_t)p->total * p->ks.real_bytes + 255) & ~size_t(255);  // normal field for HLLD
+    }
+    *bytes = b;
+    return AH_OK;
+}
+
+int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
+    if (!p || !state || (!ws && p->nregs + 1 > 0)) return AH_ERR_INVALID;
+    size_t need;
+    ah_plan_workspace_size(p, &need);
+    if (ws_bytes < need) return AH_ERR_NOMEM;
+    char* c = (char*)ws;
+    p->U = state;
+    p->regs[0] = state;
+    for (int r = 1; r <= 8; ++r) p->regs[r] = nullptr;
+    // K registers first, then saved L registers, depending on the scheme
+    int nk = 0, nl = 0;
+    switch (p->sch.rk) {
+        case AH_SSPRK22: nk = 1; break;
+        case AH_SSPRK33: nk = 2; break;
+        case AH_SSPRK43: nk = 3; break;
+        case AH_RK4: nk = 3; nl = 3; break;
+        case AH_SSPRK53: nk = 4; nl = 2; break;
+        case AH_SSPRK54: nk = 4; nl = 1; break;
+        case AH_SSPRK104: nk = 3; break;
+    }
+    for (int r = 0; r < nk; ++r) {
+        p->regs[1 + r] = (double*)c;
+        c += plane_bytes(p, 8);
+    }
+    for (int r = 0; r < nl; ++r) {
+        p->regs[5 + r] = (double*)c;
+        c += plane_bytes(p, 8);
+    }
+    p->Lbuf = (double*)c;
+    c += plane_bytes(p, 8);
+    if (p->sch.dims == 2) {
+        p->Dbuf = (double*)c;
+        c += plane_bytes(p, 8);
+        p->wL = c;
+        c += plane_bytes(p, p->ks.real_bytes);
+        p->wR = c;
+        c += plane_bytes(p, p->ks.real_bytes);
+        p->bn = c;
+    }
+    p->bound = true;
+    return AH_OK;
+}
+
+int ah_plan_reset(ah_plan* p, void* stream) {
+    if (!p) return AH_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reset_kernel, dim3((p->nprob + 255) / 256), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+    HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+static SweepArgs base_args(ah_plan* p) {
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.plane = p->total;
+    a.pd = p->d_pd;
+    a.ps = p->d_ps;
+    a.limiter = p->sch.limiter;
+    a.flattener = p->sch.flattener;
+    a.parity_override = -1;
+    return a;
+}
+
+// L of the stage input X for every active problem (evolve_space + compute_L)
+static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, int parity_override,
+                         double* flux0, double* flux1, const Tile* t1, int n1, const Tile* const* tr, const int* nr,
+                         const Tile* const* tf, const int* nf, hipStream_t s) {
+    SweepArgs a = base_args(p);
+    a.X = X;
+    a.stage0 = stage0;
+    a.parity_override = parity_override;
+    if (p->sch.dims == 1) {
+        a.out = Lout;
+        a.tiles = t1;
+        a.flux_out = flux0;
+        a.flux_plane = p->probs[0].n0 + 1;
+        p->ks.sweep1d(a, n1, s);
+        return;
+    }
+    a.wL = p->wL;
+    a.wR = p->wR;
+    a.bn = p->sch.solver == AH_HLLD ? p->bn : nullptr;
+    for (int ax = 0; ax < 2; ++ax) {
+        a.tiles = tr[ax];
+        p->ks.recon[ax](a, nr[ax], s);
+        a.tiles = tf[ax];
+        a.last = ax == 1;
+        a.out = ax == 0 ? p->Dbuf : Lout;
+        a.Dprev = p->Dbuf;
+        a.flux_out = ax == 0 ? flux0 : flux1;
+        a.flux_plane = ax == 0 ? (long long)(p->probs[0].n0 + 1) * p->probs[0].n1
+                               : (long long)(p->probs[0].n1 + 1) * p->probs[0].n0;
+        p->ks.flux[ax](a, nf[ax], s);
+    }
+}
+
+static void launch_combine(ah_plan* p, int op, hipStream_t s) {
+    CombineArgs c;
+    for (int r = 0; r < 9; ++r) c.regs[r] = p->regs[r];
+    c.L = p->Lbuf;
+    c.pd = p->d_pd;
+    c.ps = p->d_ps;
+    c.cell_prob = p->nprob > 1 ? p->d_cell_prob : nullptr;
+    c.plane = p->total;
+    c.nv = p->nv;
+    c.op = op;
+    const long long blocks = std::min<long long>((p->total + 255) / 256, 8192);
+    hipLaunchKernelGGL(combine_kernel, dim3((unsigned)blocks), dim3(256), 0, s, c);
+}
+
+int ah_plan_step(ah_plan* p, int nsteps, void* stream) {
+    if (!p || nsteps < 0) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    const Tile* tr[2] = {p->d_tr[0], p->d_tr[1]};
+    const Tile* tf[2] = {p->d_tf[0], p->d_tf[1]};
+    const int pb = (p->nprob + 255) / 256;
+    for (int n = 0; n < nsteps; ++n) {
+        for (size_t st = 0; st < p->prog.size(); ++st) {
+            const Stage& S = p->prog[st];
+            launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, p->d_t1, p->n1t, tr, p->nrt, tf,
+                         p->nft, s);
+            if (st == 0)
+                hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
+            launch_combine(p, S.op, s);
+        }
+        hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+    }
+    HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+int ah_plan_status(ah_plan* p, ah_problem_status* out, void* stream) {
+    if (!p || !out) return AH_ERR_INVALID;
+    std::vector<ProbState> h(p->nprob);
+    HIPCHK(hipMemcpyAsync(h.data(), p->d_ps, p->nprob * sizeof(ProbState), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    for (int i = 0; i < p->nprob; ++i) {
+        out[i].t = h[i].t;
+        out[i].dt = h[i].dt;
+        out[i].steps = h[i].steps;
+        out[i].active = h[i].active;
+        out[i].failed = h[i].failed;
+    }
+    return AH_OK;
+}
+
+int ah_evolve_space(ah_plan* p, int prob, int parity, const double* grid, double* flux0, double* flux1, double* eigmax,
+                    void* stream) {
+    if (!p || prob != 0 || p->nprob != 1 || !grid || !flux0 || (p->sch.dims == 2 && !flux1) || !eigmax)
+        return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    ProbState saved, tmp;
+    HIPCHK(hipMemcpyAsync(&saved, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    tmp = saved;
+    tmp.active = 1;
+    tmp.fail_flag = 0;
+    tmp.eig[0] = tmp.eig[1] = 0ull;
+    tmp.parity = parity;
+    HIPCHK(hipMemcpyAsync(p->d_ps, &tmp, sizeof(ProbState), hipMemcpyHostToDevice, s));
+    const Tile* tr[2] = {p->d_tr[0], p->d_tr[1]};
+    const Tile* tf[2] = {p->d_tf[0], p->d_tf[1]};
+    launch_space(p, grid, p->Lbuf, 1, parity, flux0, flux1, p->d_t1, p->n1t, tr, p->nrt, tf, p->nft, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&tmp, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(&eigmax[0], &tmp.eig[0], sizeof(double));
+    if (p->sch.dims == 2) memcpy(&eigmax[1], &tmp.eig[1], sizeof(double));
+    HIPCHK(hipMemcpyAsync(p->d_ps, &saved, sizeof(ProbState), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return AH_OK;
+}
+
+int ah_compute_L(ah_plan* p, int prob, int parity, const double* flux0, const double* flux1, double* L, void* stream) {
+    (void)parity;
+    if (!p || prob != 0 || p->nprob != 1 || !flux0 || !L || (p->sch.dims == 2 && !flux1)) return AH_ERR_INVALID;
+    const ah_problem& q = p->probs[0];
+    const int n1 = p->sch.dims == 1 ? 1 : q.n1;
+    const long long ncell = (long long)q.n0 * n1;
+    const long long fp0 = (long long)(q.n0 + 1) * n1, fp1 = (long long)(n1 + 1) * q.n0;
+    hipLaunchKernelGGL(fluxdiff_kernel, dim3((unsigned)std::min<long long>((ncell + 255) / 256, 4096)), dim3(256), 0,
+                       (hipStream_t)stream, flux0, flux1, L, q.n0, n1, p->sch.dims, p->nv, q.dx, p->total, fp0, fp1);
+    HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,553 @@
+// Device building blocks of the astrea finite-volume update on gfx950.
+//
+// Everything here is templated on the arithmetic type R (double for the plain
+// path, float for the split path) and on the live-component mask MASK (bit c:
+// component c of rho, v|m x,y,z, p|E, B x,y,z is carried).  Components outside
+// the mask are zero in the reference for the problem class the mask names;
+// their terms are dropped at compile time.  Dropping a zero term changes a
+// result only in the sign of a zero, so FP64 values equal the reference's
+// numpy evaluation, which writes every expression in the order used here.
+//
+// Reference formulas: functions/fv.py, functions/constructor.py,
+// num_methods/limiters.py, schemes/*.py, num_methods/solvers.py.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ah {
+
+enum { EDGE = 0, WRAP = 1 };
+enum { PCM = 0, PLM = 1, PPM = 2, WENO3 = 3, WENO5 = 4, WENO7 = 5 };
+enum { LLF = 0, HLLC = 1, HLLD = 2 };
+enum { MINMOD = 0, VANLEER = 1, OSPRE = 2, VANALBADA1 = 3, KOREN = 4, SUPERBEE = 5 };
+enum { FLAT_OFF = 0, FLAT_ASTREA = 1, FLAT_COLELLA90 = 2 };
+
+// Python float literals of the reference, rounded once to double.
+constexpr double K24 = 1.0 / 24.0;
+constexpr double K7_12 = 7.0 / 12.0;
+constexpr double K1_12 = 1.0 / 12.0;
+
+template <int MASK>
+struct VS {
+    static constexpr __host__ __device__ bool live(int c) { return (MASK >> c) & 1; }
+    static constexpr __host__ __device__ int slot(int c) {
+        int k = 0;
+        for (int i = 0; i < c; ++i) k += (MASK >> i) & 1;
+        return k;
+    }
+    static constexpr int NV = ((MASK >> 0) & 1) + ((MASK >> 1) & 1) + ((MASK >> 2) & 1) + ((MASK >> 3) & 1) +
+                              ((MASK >> 4) & 1) + ((MASK >> 5) & 1) + ((MASK >> 6) & 1) + ((MASK >> 7) & 1);
+    static constexpr bool MHD = ((MASK >> 5) & 7) != 0;
+    // component index of storage slot s
+    static constexpr __host__ __device__ int comp(int s) {
+        int k = 0;
+        for (int c = 0; c < 8; ++c)
+            if ((MASK >> c) & 1) {
+                if (k == s) return c;
+                ++k;
+            }
+        return -1;
+    }
+};
+
+template <class R>
+struct V8 {
+    R c[8];
+};
+
+template <class R, int MASK>
+__device__ inline V8<R> zero8() {
+    V8<R> z;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) z.c[i] = R(0);
+    return z;
+}
+
+// precise (IEEE) square root and absolute value for each arithmetic type
+__device__ inline double ah_sqrt(double x) { return ::sqrt(x); }
+__device__ inline float ah_sqrt(float x) { return ::sqrtf(x); }
+__device__ inline double ah_fabs(double x) { return ::fabs(x); }
+__device__ inline float ah_fabs(float x) { return ::fabsf(x); }
+__device__ inline bool ah_isfinite(double x) { return __builtin_isfinite(x); }
+__device__ inline bool ah_isfinite(float x) { return __builtin_isfinite(x); }
+
+// ---- elementary operations (functions/fv.py:19-38) --------------------------
+template <class R>
+__device__ inline R gdiv(R a, R b) { return b != R(0) ? a / b : R(0); }   // fv.divide
+template <class R>
+__device__ inline R nmin(R a, R b) { return (a < b || a != a) ? a : b; }  // np.minimum (NaN-propagating)
+template <class R>
+__device__ inline R nmax(R a, R b) { return (a > b || a != a) ? a : b; }  // np.maximum
+template <class R>
+__device__ inline R nsign(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : x); }
+template <class R>
+__device__ inline R sq(R x) { return x * x; }
+
+// ((x0^2 + x1^2) + x2^2) over components B0..B0+2, dead ones skipped
+template <class R, int MASK, int B0>
+__device__ inline R sumsq3(const V8<R>& w) {
+    using V = VS<MASK>;
+    R acc = R(0);
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (V::live(B0 + i)) {
+            R t = w.c[B0 + i] * w.c[B0 + i];
+            acc = first ? t : acc + t;
+            first = false;
+        }
+    return acc;
+}
+template <class R, int MASK, int B0>
+constexpr int nlive3() { return VS<MASK>::live(B0) + VS<MASK>::live(B0 + 1) + VS<MASK>::live(B0 + 2); }
+
+// fv.norm(x)**2: the square of the rounded norm.  With one live component
+// ah_sqrt(fl(x*x)) == |x| exactly, so the square is fl(x*x).
+template <class R, int MASK, int B0>
+__device__ inline R norm2(const V8<R>& w) {
+    if constexpr (nlive3<R, MASK, B0>() == 0) return R(0);
+    else if constexpr (nlive3<R, MASK, B0>() == 1) return sumsq3<R, MASK, B0>(w);
+    else {
+        R n = ah_sqrt(sumsq3<R, MASK, B0>(w));
+        return n * n;
+    }
+}
+// np.sum(v*B, axis=-1) over the velocity / field components that are both live
+template <class R, int MASK>
+__device__ inline R dot_vB(const V8<R>& w) {
+    using V = VS<MASK>;
+    R acc = R(0);
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (V::live(1 + i) && V::live(5 + i)) {
+            R t = w.c[1 + i] * w.c[5 + i];
+            acc = first ? t : acc + t;
+            first = false;
+        }
+    return acc;
+}
+
+// ---- variable conversions (fv.py:49-54, 89-101) ------------------------------
+template <class R, int MASK>
+__device__ inline V8<R> prim_to_cons(const V8<R>& w, R gamma) {
+    using V = VS<MASK>;
+    V8<R> q = w;
+    R e = w.c[4] / (gamma - R(1));
+    if constexpr (V::MHD)
+        q.c[4] = e + R(0.5) * (w.c[0] * norm2<R, MASK, 1>(w) + norm2<R, MASK, 5>(w));
+    else
+        q.c[4] = e + R(0.5) * (w.c[0] * norm2<R, MASK, 1>(w));
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (V::live(i)) q.c[i] = w.c[i] * w.c[0];
+    return q;
+}
+
+template <class R, int MASK>
+__device__ inline V8<R> cons_to_prim(const V8<R>& q, R gamma) {
+    using V = VS<MASK>;
+    V8<R> w = q;
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (V::live(i)) w.c[i] = gdiv(q.c[i], q.c[0]);
+    if constexpr (V::MHD)
+        w.c[4] = (gamma - R(1)) * (q.c[4] - R(0.5) * (q.c[0] * norm2<R, MASK, 1>(w) + norm2<R, MASK, 5>(q)));
+    else
+        w.c[4] = (gamma - R(1)) * (q.c[4] - R(0.5) * (q.c[0] * norm2<R, MASK, 1>(w)));
+    return w;
+}
+
+// ---- physical flux along axis AX (constructor.py:113-125) --------------------
+template <class R, int MASK, int AX>
+__device__ inline V8<R> phys_flux(const V8<R>& w, R gamma) {
+    using V = VS<MASK>;
+    constexpr int A = AX % 3, O = (AX + 1) % 3, C = (AX + 2) % 3;
+    V8<R> f = zero8<R, MASK>();
+    const R rho = w.c[0], u = w.c[AX + 1], p = w.c[4];
+    const R ru = rho * u;
+    f.c[0] = ru;
+    if constexpr (V::MHD) {
+        const R b2 = norm2<R, MASK, 5>(w);
+        const R Ba = V::live(AX + 5) ? w.c[AX + 5] : R(0);
+        f.c[A + 1] = rho * (u * u) + p + R(0.5) * b2 - Ba * Ba;
+        if (V::live(O + 1)) f.c[O + 1] = ru * w.c[O + 1] - Ba * w.c[O + 5];
+        if (V::live(C + 1)) f.c[C + 1] = ru * w.c[C + 1] - Ba * w.c[C + 5];
+        f.c[4] = u * (R(0.5) * rho * norm2<R, MASK, 1>(w) + (gamma * p) / (gamma - R(1)) + b2) - Ba * dot_vB<R, MASK>(w);
+        if (V::live(O + 5)) f.c[O + 5] = w.c[O + 5] * u - Ba * w.c[O + 1];
+        if (V::live(C + 5)) f.c[C + 5] = w.c[C + 5] * u - Ba * w.c[C + 1];
+    } else {
+        f.c[A + 1] = rho * (u * u) + p;
+        if (V::live(O + 1)) f.c[O + 1] = ru * w.c[O + 1];
+        if (V::live(C + 1)) f.c[C + 1] = ru * w.c[C + 1];
+        f.c[4] = u * (R(0.5) * rho * norm2<R, MASK, 1>(w) + (gamma * p) / (gamma - R(1)));
+    }
+    return f;
+}
+
+// ---- Roe average (constructor.py:167-176) and arithmetic mean (plm.py:84) ----
+template <class R, int MASK>
+__device__ inline V8<R> roe_avg(const V8<R>& wp, const V8<R>& wm) {
+    using V = VS<MASK>;
+    V8<R> a = zero8<R, MASK>();
+    const R rm = ah_sqrt(wm.c[0]), rp = ah_sqrt(wp.c[0]);
+    const R s = rm + rp;
+    a.c[0] = rm * rp;
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (V::live(i)) a.c[i] = gdiv(wp.c[i] * rp + wm.c[i] * rm, s);
+    a.c[4] = gdiv(rp * wp.c[4] + rm * wm.c[4], s);
+#pragma unroll
+    for (int i = 5; i < 8; ++i)
+        if (V::live(i)) a.c[i] = gdiv(wp.c[i] * rm + wm.c[i] * rp, s);
+    return a;
+}
+template <class R, int MASK>
+__device__ inline V8<R> arith_avg(const V8<R>& wp, const V8<R>& wm) {
+    V8<R> a = zero8<R, MASK>();
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (VS<MASK>::live(i)) a.c[i] = R(0.5) * (wp.c[i] + wm.c[i]);
+    return a;
+}
+
+// ---- spectral radius of the primitive Jacobian (replaces fv.py:157-161) ------
+// The B-normal row of constructor.make_Jacobian is zero, so its spectrum is
+// {0} plus that of the 7x7 primitive MHD Jacobian: u, u+-c_a, u+-c_s, u+-c_f.
+// max|lambda| = |u| + c_f.  A non-finite state returns NaN (the reference's
+// LAPACK call raises on it), which the caller folds into the failure flag.
+template <class R, int MASK, int AX>
+__device__ inline R spectral_radius(const V8<R>& w, R gamma) {
+    using V = VS<MASK>;
+    bool fin = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (V::live(i)) fin = fin && ah_isfinite(w.c[i]);
+    const R rho = w.c[0], u = w.c[AX + 1], p = w.c[4];
+    const R a2 = gamma * p / rho;
+    R r;
+    if constexpr (!V::MHD) {
+        r = a2 >= R(0) ? ah_fabs(u) + ah_sqrt(a2) : ah_sqrt(ah_fabs(u * u - a2));
+    } else {
+        const R b2 = sumsq3<R, MASK, 5>(w) / rho;
+        const R Bn = V::live(AX + 5) ? w.c[AX + 5] : R(0);
+        const R bn2 = Bn * Bn / rho;
+        const R s = a2 + b2;
+        const R disc = s * s - R(4) * a2 * bn2;
+        const R cf2 = R(0.5) * (s + ah_sqrt(disc > R(0) ? disc : R(0)));
+        r = ah_fabs(u) + ah_sqrt(cf2 > R(0) ? cf2 : R(0));
+    }
+    return fin ? r : R(NAN);
+}
+
+// ---- slope limiters (limiters.py:10-49); a backward, b forward difference ----
+template <class R>
+__device__ inline R slope_limit(int lim, R a, R b) {
+    if (lim == MINMOD) {
+        const bool ab = a * b > R(0);
+        if (ah_fabs(a) < ah_fabs(b) && ab) return a;
+        if (ah_fabs(a) >= ah_fabs(b) && ab) return b;
+        return R(0);
+    }
+    const R r = gdiv(a, b);
+    R phi;
+    switch (lim) {
+        case VANLEER: phi = (r + ah_fabs(r)) / (R(1) + ah_fabs(r)); break;
+        case OSPRE: phi = R(1.5) * ((r * r + r) / (r * r + r + R(1))); break;
+        case VANALBADA1: phi = (r * r + r) / (r * r + R(1)); break;
+        case KOREN: phi = nmax(R(0), nmin(nmin(R(2) * r, (R(2) + r) / R(3)), R(2))); break;
+        default: phi = nmax(R(0), nmax(nmin(R(2) * r, R(1)), nmin(r, R(2)))); break;
+    }
+    return phi * b;
+}
+
+// ---- PPM 'mc' extrapolant limiter, one variable (limiters.py:82-140) --------
+// The reference's grid-global cell_extrema.any() test (limiters.py:102) is
+// always true (the first cell with edge ghosts, or any periodic line, has a
+// cell extremum), so only that branch exists here.
+// Inputs: wS, neighbours wm1/wp1/wm2/wp2, face values fL/fR, curvatures
+// d2 at i-1, i, i+1 and third differences d3 at i-2..i+2.
+template <class R>
+__device__ inline void mc_limit(R wS, R wm1, R wp1, R wm2, R wp2, R fL, R fR, R d2m, R d2c, R d2p, R d3m2, R d3m1,
+                                R d30, R d3p2, R& wL, R& wR) {
+    const R C = R(1.25);
+    const R dwm = wS - fL, dwp = fR - wS;
+    wL = fL;
+    wR = fR;
+    const R d2w = R(6) * (fL - R(2) * wS + fR);
+    const bool cell_extrema = (dwm * dwp <= R(0)) || ((wS - wm2) * (wp2 - wS) <= R(0));
+    const R lim_curv = nsign(d2c) * nmin(nmin(ah_fabs(d2w), C * ah_fabs(d2c)), nmin(C * ah_fabs(d2p), C * ah_fabs(d2m)));
+    const R d2w_lim = cell_extrema ? lim_curv : R(0);
+    const bool sens = ah_fabs(d2w) > R(1e-12) * nmax(ah_fabs(wS), nmax(nmax(ah_fabs(wm1), ah_fabs(wp1)), nmax(ah_fabs(wm2), ah_fabs(wp2))));
+    const R rho = sens ? gdiv(d2w_lim, d2w) : R(0);
+    const R d3min = nmin(nmin(d3m1, d30), nmin(d3m2, d3p2));
+    const R d3max = nmax(nmax(d3m1, d30), nmax(d3m2, d3p2));
+    const bool roundoff = (rho < R(1.0 - 1e-12)) || (R(0.1) * nmax(ah_fabs(d3max), ah_fabs(d3min)) <= (d3max - d3min));
+    if ((dwm * dwp < R(0)) && roundoff) {
+        wL = wS - rho * dwm;
+        wR = wS + rho * dwp;
+    }
+    if (roundoff && ah_fabs(dwm) >= R(2) * ah_fabs(dwp)) wL = wS - R(2) * (R(1) - rho) * dwp - rho * dwm;
+    if (roundoff && ah_fabs(dwp) >= R(2) * ah_fabs(dwm)) wR = wS + R(2) * (R(1) - rho) * dwm + rho * dwp;
+}
+
+// ---- WENO (weno.py:22-149), component-wise, eps = 1e-6 -----------------------
+template <class R>
+__device__ inline void weno3(R m1, R z, R p1, R& wL, R& wR) {
+    const R eps = R(1e-6);
+    const R b0 = sq(z - m1), b1 = sq(p1 - z);
+    const R g0 = R(1.0 / 3.0), g1 = R(2.0 / 3.0);
+    const R e0 = sq(b0 + eps), e1 = sq(b1 + eps);
+    const R a0g0 = g0 / e0, a1g1 = g1 / e1, a1g0 = g0 / e1, a0g1 = g1 / e0;
+    wR = (a0g0 / (a0g0 + a1g1)) * (R(1.5) * z - R(0.5) * m1) + (a1g1 / (a0g0 + a1g1)) * (R(0.5) * z + R(0.5) * p1);
+    wL = (a1g0 / (a0g1 + a1g0)) * (R(1.5) * z - R(0.5) * p1) + (a0g1 / (a0g1 + a1g0)) * (R(0.5) * z + R(0.5) * m1);
+}
+
+template <class R>
+__device__ inline void weno5(R m2, R m1, R z, R p1, R p2, R& wL, R& wR) {
+    const R eps = R(1e-6);
+    const R c13 = R(13.0 / 12.0), c14 = R(1.0 / 4.0);
+    const R b0 = c13 * sq(m2 - R(2) * m1 + z) + c14 * sq(m2 - R(4) * m1 + R(3) * z);
+    const R b1 = c13 * sq(m1 - R(2) * z + p1) + c14 * sq(m1 - p1);
+    const R b2 = c13 * sq(z - R(2) * p1 + p2) + c14 * sq(R(3) * z - R(4) * p1 + p2);
+    const R g0 = R(1.0 / 10.0), g1 = R(3.0 / 5.0), g2 = R(3.0 / 10.0);
+    const R e0 = sq(b0 + eps), e1 = sq(b1 + eps), e2 = sq(b2 + eps);
+    const R A0g0 = g0 / e0, A1g1 = g1 / e1, A2g2 = g2 / e2, A0g2 = g2 / e0, A2g0 = g0 / e2;
+    const R t3 = R(1.0 / 3.0), t76 = R(7.0 / 6.0), t116 = R(11.0 / 6.0), m16 = R(-1.0 / 6.0), t56 = R(5.0 / 6.0);
+    const R sR = A0g0 + A1g1 + A2g2;
+    wR = (A0g0 / sR) * (t3 * m2 - t76 * m1 + t116 * z) + (A1g1 / sR) * (m16 * m1 + t56 * z + t3 * p1) +
+         (A2g2 / sR) * (t3 * z + t56 * p1 - R(1.0 / 6.0) * p2);
+    const R sL = A0g2 + A1g1 + A2g0;
+    wL = (A0g2 / sL) * (t3 * z + t56 * m1 - R(1.0 / 6.0) * m2) + (A1g1 / sL) * (m16 * p1 + t56 * z + t3 * m1) +
+         (A2g0 / sL) * (t3 * p2 - t76 * p1 + t116 * z);
+}
+
+template <class R>
+__device__ inline void weno7(R m3, R m2, R m1, R z, R p1, R p2, R p3, R& wL, R& wR) {
+    const R eps = R(1e-6);
+    const R b0 = m3 * (R(547) * m3 - R(3882) * m2 + R(4642) * m1 - R(1854) * z) +
+                 m2 * (R(7043) * m2 - R(17246) * m1 + R(7042) * z) + m1 * (R(11003) * m1 - R(9402) * z) + z * (R(2107) * z);
+    const R b1 = m2 * (R(267) * m2 - R(1642) * m1 + R(1602) * z - R(494) * p1) +
+                 m1 * (R(2843) * m1 - R(5966) * z + R(1922) * p1) + z * (R(3443) * z - R(2522) * p1) + p1 * (R(547) * p1);
+    const R b2 = m1 * (R(547) * m1 - R(2522) * z + R(1922) * p1 - R(494) * p2) +
+                 z * (R(3443) * z - R(5966) * p1 + R(1602) * p2) + p1 * (R(2843) * p1 - R(1642) * p2) + p2 * (R(267) * p2);
+    const R b3 = z * (R(2107) * z - R(9402) * p1 + R(7042) * p2 - R(1854) * p3) +
+                 p1 * (R(11003) * p1 - R(17246) * p2 + R(4642) * p3) + p2 * (R(7043) * p2 - R(3882) * p3) + p3 * (R(547) * p3);
+    const R g0 = R(1.0 / 35.0), g1 = R(12.0 / 35.0), g2 = R(18.0 / 35.0), g3 = R(4.0 / 35.0);
+    const R e0 = sq(b0 + eps), e1 = sq(b1 + eps), e2 = sq(b2 + eps), e3 = sq(b3 + eps);
+    const R q14 = R(1.0 / 4.0), q1312 = R(13.0 / 12.0), q2312 = R(23.0 / 12.0), q2512 = R(25.0 / 12.0),
+            q112 = R(1.0 / 12.0), q512 = R(5.0 / 12.0), q712 = R(7.0 / 12.0);
+    {
+        const R A0 = g0 / e0, A1 = g1 / e1, A2 = g2 / e2, A3 = g3 / e3;
+        const R s = A0 + A1 + A2 + A3;
+        wR = (A0 / s) * (R(-1.0 / 4.0) * m3 + q1312 * m2 - q2312 * m1 + q2512 * z) +
+             (A1 / s) * (q112 * m2 - q512 * m1 + q1312 * z + q14 * p1) +
+             (A2 / s) * (R(-1.0 / 12.0) * m1 + q712 * z + q712 * p1 - q112 * p2) +
+             (A3 / s) * (q14 * z + q1312 * p1 - q512 * p2 + q112 * p3);
+    }
+    {
+        const R A0 = g3 / e0, A1 = g2 / e1, A2 = g1 / e2, A3 = g0 / e3;
+        const R s = A0 + A1 + A2 + A3;
+        wL = (A0 / s) * (q14 * z + q1312 * m1 - q512 * m2 + q112 * m3) +
+             (A1 / s) * (R(-1.0 / 12.0) * p1 + q712 * z + q712 * m1 - q112 * m2) +
+             (A2 / s) * (q112 * p2 - q512 * p1 + q1312 * z + q14 * m1) +
+             (A3 / s) * (R(-1.0 / 4.0) * p3 + q1312 * p2 - q2312 * p1 + q2512 * z);
+    }
+}
+
+// ---- PPM slope flattener (schemes/ppm.py:111-134, Colella 1990) -------------
+// chi~ of one cell from pressure p(i-2..i+2) and normal velocity u(i-1), u(i+1).
+template <class R>
+__device__ inline R flat_chi(int variant, R pm2, R pm1, R pp1, R pp2, R um1, R up1) {
+    const R delta = R(0.33), z0 = R(0.75), z1 = R(0.85);
+    const R dp1 = pp1 - pm1;
+    const R z = gdiv(ah_fabs(dp1), ah_fabs(pp2 - pm2));
+    R chi = R(1) - gdiv(z - z0, R(0.85 - 0.75));
+    if (z > z1) chi = R(0);
+    if (z < z0) chi = R(1);
+    const bool weak = gdiv(ah_fabs(dp1), nmin(pp1, pm1)) <= delta;
+    const bool expanding = (um1 - up1) <= R(0);
+    if (variant == FLAT_ASTREA) {
+        if (expanding && weak) chi = R(0);
+    } else {
+        if (expanding || weak) chi = R(1);
+    }
+    return chi;
+}
+
+// ---- Riemann solvers (solvers.py) --------------------------------------------
+// Interface quantities: w/q/f for the minus (left) and plus (right) states.
+template <class R>
+struct Iface {
+    V8<R> wm, wp, qm, qp, fm, fp;
+};
+
+// Local Lax-Friedrichs (solvers.py:69-75)
+template <class R, int MASK>
+__device__ inline V8<R> flux_llf(const Iface<R>& s, R lam) {
+    V8<R> F = zero8<R, MASK>();
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (VS<MASK>::live(i)) F.c[i] = R(0.5) * (s.fm.c[i] + s.fp.c[i]) - R(0.5) * ((s.qp.c[i] - s.qm.c[i]) * lam);
+    return F;
+}
+
+// HLLC (solvers.py:92-138).  sax = the solver's axis argument (the
+// enumeration index of the sweep, solvers.py:34-36); the star momentum is
+// written at index 1 whatever the axis (127-128); F(w+) is the default (133).
+template <class R, int MASK>
+__device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma) {
+    using V = VS<MASK>;
+    const R rhoL = s.wm.c[0], pL = s.wm.c[4], rhoR = s.wp.c[0], pR = s.wp.c[4];
+    const R uL = sax == 0 ? s.wm.c[1] : (V::live(2) ? s.wm.c[2] : R(0));
+    const R uR = sax == 0 ? s.wp.c[1] : (V::live(2) ? s.wp.c[2] : R(0));
+    const R cL = ah_sqrt(gamma * gdiv(pL, rhoL)), cR = ah_sqrt(gamma * gdiv(pR, rhoR));
+    const R sqL = ah_sqrt(rhoL), sqR = ah_sqrt(rhoR);
+    const R ssum = sqL + sqR;
+    const R u_hat = gdiv(uL * sqL + uR * sqR, ssum);
+    const R c2_hat = gdiv(sqL * (cL * cL) + sqR * (cR * cR), ssum) + R(0.5) * sq(uR - uL) * gdiv(sqL * sqR, ssum * ssum);
+    const R sc = ah_sqrt(c2_hat);
+    const R sL = nmin(uL - cL, u_hat - sc);
+    const R sR = nmax(uR + cR, u_hat + sc);
+    const R sM = gdiv(pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR), rhoL * (sL - uL) - rhoR * (sR - uR));
+    const R kL = gdiv(sL - uL, sL - sM), kR = gdiv(sR - uR, sR - sM);
+    const bool useL = (sL <= R(0)) && (R(0) < sM);
+    const bool useR = (sM <= R(0)) && (R(0) <= sR);
+    const bool useP = sR < R(0);
+    // select first, then evaluate one star flux: F(w+) | F*L | F*R | F(w+)
+    V8<R> F = s.fp;
+    if (useP || !(useL || useR)) return F;
+    const bool left = useL && !useR;
+    const V8<R>& Q = left ? s.qm : s.qp;
+    const V8<R>& Fb = left ? s.fm : s.fp;
+    const R k = left ? kL : kR, rho = left ? rhoL : rhoR, u = left ? uL : uR, p = left ? pL : pR;
+    const R sS = left ? sL : sR;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (!V::live(i)) continue;
+        R qs = Q.c[i] * k;
+        if (i == 1) qs = rho * k * sM;
+        if (i == 4) qs = qs + k * (sM - u) * (rho * sM + gdiv(p, sS - u));
+        F.c[i] = Fb.c[i] + (qs - Q.c[i]) * sS;
+    }
+    return F;
+}
+
+// HLLD (solvers.py:142-232), evaluated on all 8 components exactly as the
+// reference writes it (dead components enter as zeros).  SAX = solver axis;
+// Bn = normal field of the right cell's centred state (solvers.py:168,183).
+template <class R, int MASK, int SAX>
+__device__ inline V8<R> flux_hlld(const Iface<R>& s, R Bn, R gamma) {
+    constexpr int A = SAX % 3, O = (SAX + 1) % 3, C = (SAX + 2) % 3;
+    auto cfast = [&](const V8<R>& w) {
+        const R rho = w.c[0], p = w.c[4];
+        const R cs = ah_sqrt(gdiv(gamma * p, rho));
+        const R ca = gdiv(ah_sqrt((w.c[5] * w.c[5] + w.c[6] * w.c[6]) + w.c[7] * w.c[7]), ah_sqrt(rho));
+        const R cax = gdiv(w.c[5], ah_sqrt(rho));
+        const R cs2 = cs * cs, ca2 = ca * ca;
+        return ah_sqrt(R(0.5) * (cs2 + ca2 + ah_sqrt(sq(cs2 + ca2) - R(4) * cs2 * (cax * cax))));
+    };
+    auto n2 = [](R a, R b, R c) {
+        R n = ah_sqrt((a * a + b * b) + c * c);
+        return n * n;
+    };
+    auto d3 = [](R a0, R b0, R a1, R b1, R a2, R b2) { return (a0 * b0 + a1 * b1) + a2 * b2; };
+    const V8<R>&wL = s.wm, &wR = s.wp, &QL = s.qm, &QR = s.qp;
+    const R rhoL = wL.c[0], pL = wL.c[4], rhoR = wR.c[0], pR = wR.c[4];
+    const R uL = wL.c[SAX + 1], uR = wR.c[SAX + 1];
+    const R cfL = cfast(wL), cfR = cfast(wR);
+    const R umin = nmin(uL, uR), cmax = nmax(cfL, cfR);
+    const R sL = umin - cmax, sR = umin + cmax;
+    const R bL2 = n2(wL.c[5], wL.c[6], wL.c[7]), bR2 = n2(wR.c[5], wR.c[6], wR.c[7]);
+    const R den = rhoL * (sL - uL) - rhoR * (sR - uR);
+    const R sM = gdiv(pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR) + R(0.5) * bR2 - R(0.5) * bL2, den);
+    const R rLs = rhoL * gdiv(sL - uL, sL - sM), rRs = rhoR * gdiv(sR - uR, sR - sM);
+    const R sLs = sM - gdiv(wL.c[SAX + 5], ah_sqrt(rLs)), sRs = sM - gdiv(wR.c[SAX + 5], ah_sqrt(rRs));
+    const R ps = gdiv(rhoL * (pR + R(0.5) * bR2) * (sL - uL) - rhoR * (pL + R(0.5) * bL2) * (sR - uR) +
+                          rhoR * rhoL * (sL - uL) * (sR - uR),
+                      den);
+    const R Bn2 = Bn * Bn;
+    const R dL = rhoL * (sL - uL) * (sL - sM) - Bn2, dR = rhoR * (sR - uR) * (sR - sM) - Bn2;
+    const R vyLs = wL.c[O + 1] - Bn * wL.c[O + 5] * gdiv(sM - uL, dL);
+    const R vyRs = wR.c[O + 1] - Bn * wR.c[O + 5] * gdiv(sM - uR, dR);
+    const R vzLs = wL.c[C + 1] - Bn * wL.c[C + 5] * gdiv(sM - uL, dL);
+    const R vzRs = wR.c[C + 1] - Bn * wR.c[C + 5] * gdiv(sM - uR, dR);
+    const R ByLs = wL.c[O + 5] * gdiv(rhoL * sq(sL - uL) - Bn2, dL);
+    const R ByRs = wR.c[O + 5] * gdiv(rhoR * sq(sR - uR) - Bn2, dR);
+    const R BzLs = wL.c[C + 5] * gdiv(rhoL * sq(sL - uL) - Bn2, dL);
+    const R BzRs = wR.c[C + 5] * gdiv(rhoR * sq(sR - uR) - Bn2, dR);
+    V8<R> QLs, QRs;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) QLs.c[i] = QRs.c[i] = R(0);
+    QLs.c[0] = rLs;
+    QRs.c[0] = rRs;
+    QLs.c[A + 1] = rhoL * sM;
+    QRs.c[A + 1] = rhoR * sM;
+    QLs.c[O + 1] = rhoL * vyLs;
+    QRs.c[O + 1] = rhoR * vyRs;
+    QLs.c[C + 1] = rhoL * vzLs;
+    QRs.c[C + 1] = rhoR * vzRs;
+    QLs.c[A + 5] = QL.c[A + 5];
+    QRs.c[A + 5] = QR.c[A + 5];
+    QLs.c[O + 5] = ByLs;
+    QRs.c[O + 5] = ByRs;
+    QLs.c[C + 5] = BzLs;
+    QRs.c[C + 5] = BzRs;
+    const R vBL = d3(wL.c[1], wL.c[5], wL.c[2], wL.c[6], wL.c[3], wL.c[7]);
+    const R vBR = d3(wR.c[1], wR.c[5], wR.c[2], wR.c[6], wR.c[3], wR.c[7]);
+    const R mBLs = d3(QLs.c[1], QLs.c[5], QLs.c[2], QLs.c[6], QLs.c[3], QLs.c[7]);
+    const R mBRs = d3(QRs.c[1], QRs.c[5], QRs.c[2], QRs.c[6], QRs.c[3], QRs.c[7]);
+    QLs.c[4] = gdiv(QL.c[4] * (sL - uL) - uL * (pL + R(0.5) * bL2) + ps * sM + Bn * (vBL - mBLs), sL - sM);
+    QRs.c[4] = gdiv(QR.c[4] * (sR - uR) - uR * (pR + R(0.5) * bR2) + ps * sM + Bn * (vBR - mBRs), sR - sM);
+    const R sg = nsign(Bn);
+    const R srL = ah_sqrt(rLs), srR = ah_sqrt(rRs), srs = srL + srR, srp = ah_sqrt(rRs * rLs);
+    const R vyss = gdiv(vyRs * srR + vyLs * srL + sg * (ByLs - ByRs), srs);
+    const R vzss = gdiv(vzRs * srR + vzLs * srL + sg * (BzLs - BzRs), srs);
+    const R Byss = gdiv(ByLs * srR + ByRs * srL + sg * (vyLs - vyRs) * srp, srs);
+    const R Bzss = gdiv(BzLs * srR + BzRs * srL + sg * (vzLs - vzRs) * srp, srs);
+    V8<R> QLss, QRss;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) QLss.c[i] = QRss.c[i] = R(0);
+    QLss.c[0] = rLs;
+    QRss.c[0] = rRs;
+    QLss.c[A + 1] = sM;
+    QRss.c[A + 1] = sM;
+    QLss.c[O + 1] = vyss;
+    QRss.c[O + 1] = vyss;
+    QLss.c[C + 1] = vzss;
+    QRss.c[C + 1] = vzss;
+    QLss.c[A + 5] = QLs.c[A + 5];
+    QRss.c[A + 5] = QRs.c[A + 5];
+    QLss.c[O + 5] = Byss;
+    QRss.c[O + 5] = Byss;
+    QLss.c[C + 5] = Bzss;
+    QRss.c[C + 5] = Bzss;
+    const R mBLss = d3(QLss.c[1], QLss.c[5], QLss.c[2], QLss.c[6], QLss.c[3], QLss.c[7]);
+    const R mBRss = d3(QRss.c[1], QRss.c[5], QRss.c[2], QRss.c[6], QRss.c[3], QRss.c[7]);
+    QLss.c[4] = QLs.c[4] - srL * sg * (mBLs - mBLss);
+    QRss.c[4] = QRs.c[4] - srR * sg * (mBRs - mBRss);
+    // region select (solvers.py:226-231), later masks overwrite earlier ones
+    int reg = 0;  // 0: F(w-)
+    if ((sL <= R(0)) && (R(0) < sLs)) reg = 1;
+    if ((sLs <= R(0)) && (R(0) < sM)) reg = 2;
+    if ((sM <= R(0)) && (R(0) < sRs)) reg = 3;
+    if ((sRs <= R(0)) && (R(0) <= sR)) reg = 4;
+    if (sR < R(0)) reg = 5;
+    V8<R> F;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        R v;
+        switch (reg) {
+            case 1: v = s.fm.c[i] + (QLs.c[i] - QL.c[i]) * sL; break;
+            case 2: v = s.fm.c[i] + (QLss.c[i] - QLs.c[i]) * sLs; break;
+            case 3: v = s.fp.c[i] + (QRss.c[i] - QRs.c[i]) * sRs; break;
+            case 4: v = s.fp.c[i] + (QRs.c[i] - QR.c[i]) * sR; break;
+            case 5: v = s.fp.c[i]; break;
+            default: v = s.fm.c[i]; break;
+        }
+        F.c[i] = v;
+    }
+    return F;
+}
+
+}  // namespace ah

@@ -1,0 +1,828 @@
+// Kernels of the astrea finite-volume update on gfx950 (wave64, 256-thread
+// workgroups).  See ah_device.h for the per-cell arithmetic.
+//
+// Index conventions (reference Appendix "sweep frame", schemes/*.py):
+//   s = index along the sweep axis AX, t = index along the other axis.
+//   Interface k in [0, n_s] separates cells k-1 and k.
+//   w_minus[k] = wR[bc(k-1)], w_plus[k] = wL[bc(k)] (schemes/plm.py:81).
+//   Every array the reference pads (fv.add_boundary, functions/fv.py:57-61) is
+//   evaluated at e(p) = clamp(p, 0, n-1) for edge boundaries; for periodic
+//   boundaries values are computed in place from wrapped loads (translation
+//   invariance), so e(p) = p inside a tile window.
+#pragma once
+#include "ah_device.h"
+
+namespace ah {
+
+struct ProbDesc {
+    int n0, n1, bc, nprob_pad;
+    long long off;
+    double dx, gamma, cfl, t_end, chkpt;
+    long long max_steps;
+};
+
+struct ProbState {
+    double t, dt;
+    long long steps;
+    int idx, active, failed, parity;
+    unsigned long long eig[2];  // bit pattern of max spectral radius per axis (stage 0)
+    int fail_flag, pad;
+};
+
+struct Tile {
+    int prob, s0, t0, pad;
+};
+
+struct SweepArgs {
+    const double* X;      // stage input state, packed SoA (plane stride `plane`)
+    double* out;          // 1D: L;  2D flux: D (first sweep) or L (second)
+    const double* Dprev;  // 2D second sweep: D of the first sweep
+    void* wL;             // 2D interface states, sweep-frame packed SoA, type R
+    void* wR;
+    void* bn;             // 2D HLLD: centred normal field per cell (type R)
+    double* flux_out;     // optional: interface fluxes (parity entry point)
+    long long plane;
+    long long flux_plane;
+    const ProbDesc* pd;
+    ProbState* ps;
+    const Tile* tiles;
+    int ntiles;
+    int stage0;   // accumulate the eigmax for dt (stage 0 of a step)
+    int limiter;  // PLM slope limiter
+    int flattener;
+    int parity_override;  // -1: use per-problem parity
+    int last;             // 2D: this flux launch finishes L
+};
+
+__device__ inline int clampi(int p, int lo, int hi) { return p < lo ? lo : (p > hi ? hi : p); }
+__device__ inline int bcmap(int p, int n, int bc) {
+    if (bc == WRAP) {
+        int r = p % n;
+        return r < 0 ? r + n : r;
+    }
+    return clampi(p, 0, n - 1);
+}
+
+__device__ inline void atomic_max_pos(unsigned long long* a, double v) {
+    // non-negative doubles order like their bit patterns; NaN bit patterns are
+    // larger than +inf so a NaN wins and flags the failure.
+    atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+template <class R, int MASK>
+__device__ inline V8<R> load8(const R* base, long long stride, int idx) {
+    using V = VS<MASK>;
+    V8<R> w = zero8<R, MASK>();
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (V::live(c)) w.c[c] = base[V::slot(c) * stride + idx];
+    return w;
+}
+template <class R, int MASK>
+__device__ inline void store8(R* base, long long stride, int idx, const V8<R>& w) {
+    using V = VS<MASK>;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (V::live(c)) base[V::slot(c) * stride + idx] = w.c[c];
+}
+template <class D, class S, int MASK>
+__device__ inline V8<D> cast8(const V8<S>& a) {
+    V8<D> b;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) b.c[c] = VS<MASK>::live(c) ? D(a.c[c]) : D(0);
+    return b;
+}
+
+// 4th-order cell-average conversion q -> w (functions/fv.py:126-143), FP64:
+// w = C(q - K24 lap0 q - K24 lap1 q) + (K24 lap0 C(q) + K24 lap1 C(q)),
+// C = pointwise cons->prim.  lap_a x = (x[+1] - x[0]) - (x[0] - x[-1]).
+template <int MASK, int DIM>
+__device__ inline V8<double> ho_cell_prim(const V8<double>& q, const V8<double>& qs0m, const V8<double>& qs0p,
+                                          const V8<double>& qt0m, const V8<double>& qt0p, const V8<double>& cq,
+                                          const V8<double>& cs0m, const V8<double>& cs0p, const V8<double>& ct0m,
+                                          const V8<double>& ct0p, double gamma) {
+    using V = VS<MASK>;
+    V8<double> qq = q, w = zero8<double, MASK>();
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (!V::live(c)) continue;
+        // axis order of the reference loop (ax = 0 then 1) in the array the
+        // conversion receives, i.e. the sweep frame: first s then t.
+        qq.c[c] = q.c[c] - K24 * ((qs0p.c[c] - q.c[c]) - (q.c[c] - qs0m.c[c]));
+        double acc = K24 * ((cs0p.c[c] - cq.c[c]) - (cq.c[c] - cs0m.c[c]));
+        if constexpr (DIM == 2) {
+            qq.c[c] = qq.c[c] - K24 * ((qt0p.c[c] - q.c[c]) - (q.c[c] - qt0m.c[c]));
+            acc = acc + K24 * ((ct0p.c[c] - cq.c[c]) - (cq.c[c] - ct0m.c[c]));
+        }
+        w.c[c] = acc;
+    }
+    V8<double> r = cons_to_prim<double, MASK>(qq, gamma);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (V::live(c)) r.c[c] = r.c[c] + w.c[c];
+    return r;
+}
+
+// One cell's reconstruction from a window accessor W(m) that returns the
+// primitive state at sweep position m (already boundary mapped).  `ce` is the
+// evaluation position of the cell.
+template <class R, int MASK, int RECON, int AX, class Acc>
+__device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flattener, V8<R>& wL, V8<R>& wR) {
+    using V = VS<MASK>;
+    wL = zero8<R, MASK>();
+    wR = zero8<R, MASK>();
+    if constexpr (RECON == PCM) {
+        wL = wR = W(ce);
+    } else if constexpr (RECON == PLM) {
+        const V8<R> a = W(ce - 1), z = W(ce), b = W(ce + 1);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!V::live(c)) continue;
+            const R g = R(0.5) * slope_limit(limiter, z.c[c] - a.c[c], b.c[c] - z.c[c]);
+            wL.c[c] = z.c[c] - g;
+            wR.c[c] = z.c[c] + g;
+        }
+    } else if constexpr (RECON == PPM) {
+        // window values at ce-4 .. ce+4 (W applies the boundary rule)
+        V8<R> w[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = W(ce - 4 + k);
+        // face interpolant wF at ce-1 and ce: wF[i] = 7/12 (w_i + w_i+1) - 1/12 (w_i-1 + w_i+2)
+        // (ppm.py:38); the reference pads wF itself, so wF at position m is
+        // evaluated at W.e(m) -- W.at_e handles that.
+        auto wF_at = [&](int m) {
+            const int me = W.e(m);
+            const V8<R> a = W(me - 1), b = W(me), c1 = W(me + 1), d = W(me + 2);
+            V8<R> f = zero8<R, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) f.c[c] = R(K7_12) * (b.c[c] + c1.c[c]) - R(K1_12) * (a.c[c] + d.c[c]);
+            if (flattener != FLAT_OFF) {
+                // eta(me) = min(chi(me), chi(me -/+ 1)) by sign of the pressure jump (ppm.py:127-131)
+                auto chi_at = [&](int mm) {
+                    const int e2 = W.e(mm);
+                    const V8<R> p2 = W(e2 - 2), p1 = W(e2 - 1), q1 = W(e2 + 1), q2 = W(e2 + 2);
+                    return flat_chi<R>(flattener, p2.c[4], p1.c[4], q1.c[4], q2.c[4], p1.c[AX + 1], q1.c[AX + 1]);
+                };
+                const R chi0 = chi_at(me);
+                const R sg = nsign(c1.c[4] - a.c[4]);
+                R eta = chi0;
+                if (sg < R(0)) eta = nmin(chi0, chi_at(me + 1));
+                if (sg > R(0)) eta = nmin(chi0, chi_at(me - 1));
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) f.c[c] = f.c[c] * eta + b.c[c] * (R(1) - eta);
+            }
+            return f;
+        };
+        auto d2_at = [&](int m) {
+            const int me = W.e(m);
+            const V8<R> a = W(me - 1), b = W(me), c1 = W(me + 1);
+            V8<R> d = zero8<R, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) d.c[c] = a.c[c] - R(2) * b.c[c] + c1.c[c];
+            return d;
+        };
+        const V8<R> fL = wF_at(ce - 1), fR = wF_at(ce);
+        // curvatures d2w_C padded by one (limiters.py:93,103): at e(ce-1), ce, e(ce+1)
+        const V8<R> d2m = d2_at(ce - 1), d2c = d2_at(ce), d2p = d2_at(ce + 1);
+        // third differences d3w[m] = d2w_C[e(m+1)] - d2w_C[m] (limiters.py:96), padded
+        // by two (126-127): needed at e(ce-2), e(ce-1), ce, e(ce+2)
+        auto d3_at = [&](int m) {
+            const int me = W.e(m);
+            const V8<R> hi = d2_at(me + 1), lo = d2_at(me);
+            V8<R> d = zero8<R, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) d.c[c] = hi.c[c] - lo.c[c];
+            return d;
+        };
+        const V8<R> d3m2 = d3_at(ce - 2), d3m1 = d3_at(ce - 1), d30 = d3_at(ce), d3p2 = d3_at(ce + 2);
+        const V8<R>& z = w[4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!V::live(c)) continue;
+            mc_limit<R>(z.c[c], w[3].c[c], w[5].c[c], w[2].c[c], w[6].c[c], fL.c[c], fR.c[c], d2m.c[c], d2c.c[c],
+                        d2p.c[c], d3m2.c[c], d3m1.c[c], d30.c[c], d3p2.c[c], wL.c[c], wR.c[c]);
+        }
+    } else {
+        constexpr int H = RECON == WENO3 ? 1 : (RECON == WENO5 ? 2 : 3);
+        V8<R> w[7];
+#pragma unroll
+        for (int k = 0; k < 2 * H + 1; ++k) w[k] = W(ce - H + k);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!V::live(c)) continue;
+            if constexpr (RECON == WENO3)
+                weno3<R>(w[0].c[c], w[1].c[c], w[2].c[c], wL.c[c], wR.c[c]);
+            else if constexpr (RECON == WENO5)
+                weno5<R>(w[0].c[c], w[1].c[c], w[2].c[c], w[3].c[c], w[4].c[c], wL.c[c], wR.c[c]);
+            else
+                weno7<R>(w[0].c[c], w[1].c[c], w[2].c[c], w[3].c[c], w[4].c[c], w[5].c[c], w[6].c[c], wL.c[c],
+                         wR.c[c]);
+        }
+    }
+}
+
+template <class R, int MASK, int SOLVER, int AX>
+__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma) {
+    if constexpr (SOLVER == LLF)
+        return flux_llf<R, MASK>(s, lam);
+    else if constexpr (SOLVER == HLLC)
+        return flux_hllc<R, MASK>(s, sax, gamma);
+    else
+        return sax == 0 ? flux_hlld<R, MASK, 0>(s, Bn, gamma) : flux_hlld<R, MASK, 1>(s, Bn, gamma);
+}
+
+// Block-wide max of a non-negative double, then one atomic per block.
+__device__ inline void block_max_atomic(double v, unsigned long long* dst, double* red) {
+    // NaN-aware via bit patterns: reduce as unsigned long long
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o > b ? o : b;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    unsigned long long* r = reinterpret_cast<unsigned long long*>(red);
+    if (lane == 0) r[wid] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = r[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = r[i] > m ? r[i] : m;
+        if (m != 0ull) atomicMax(dst, m);
+    }
+}
+
+// ============================================================================
+// 1D: one fused kernel per stage: load -> cons->prim -> reconstruct/limit ->
+// spectral radius -> Riemann -> flux difference -> L = -dF/dx.
+// ============================================================================
+template <class R, int MASK, int RECON, int SOLVER, int TS>
+__global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
+    using V = VS<MASK>;
+    constexpr int NV = V::NV;
+    constexpr bool HO = (RECON == PPM || RECON >= WENO3);
+    constexpr int HW_L = 5, HW_R = 6;                 // wS window [s0-5, s0+TS+6)
+    constexpr int NQ = TS + HW_L + HW_R + 2;           // q window [s0-6, s0+TS+7)
+    constexpr int NW = TS + HW_L + HW_R;
+    constexpr int NC = TS + 3;                         // cells [s0-1, s0+TS+2)
+    constexpr int NF = TS + 1;                         // interfaces [s0, s0+TS]
+    constexpr int NR = TS + 2;                         // radii at [s0, s0+TS+1]
+    __shared__ double qd[NV][NQ];
+    __shared__ double cq[HO ? NV : 1][HO ? NQ : 1];
+    __shared__ R ws[NV][NW];
+    __shared__ R wl[RECON == PCM ? 1 : NV][RECON == PCM ? 1 : NC];
+    __shared__ R wr[RECON == PCM ? 1 : NV][RECON == PCM ? 1 : NC];
+    __shared__ R rad[NR];
+    __shared__ R fl[NV][NF];
+    __shared__ double red[4];
+
+    const Tile tile = a.tiles[blockIdx.x];
+    const ProbDesc pd = a.pd[tile.prob];
+    ProbState* ps = a.ps + tile.prob;
+    if (!ps->active) return;
+    const int N = pd.n0, bc = pd.bc, s0 = tile.s0;
+    const double gamma = pd.gamma;
+    const R gR = R(gamma);
+    const long long off = pd.off, P = a.plane;
+    const int sax = 0;
+    auto e = [&](int p) { return bc == EDGE ? clampi(p, 0, N - 1) : p; };
+    const int bq = s0 - HW_L - 1, bw = s0 - HW_L, bcl = s0 - 1;
+
+    // A: load the stage input (conservative) with boundary mapping
+    for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
+        const int g = bcmap(bq + i, N, bc);
+        V8<double> q = zero8<double, MASK>();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) q.c[c] = a.X[V::slot(c) * P + off + g];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) qd[V::slot(c)][i] = q.c[c];
+        if constexpr (HO) {
+            const V8<double> w = cons_to_prim<double, MASK>(q, gamma);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) cq[V::slot(c)][i] = w.c[c];
+        }
+    }
+    __syncthreads();
+    // B: primitive cell averages wS on the window (FP64, then cast)
+    for (int i = threadIdx.x; i < NW; i += blockDim.x) {
+        const int iq = e(bw + i) - bq;
+        V8<double> w;
+        if constexpr (HO) {
+            auto Q = [&](int j) {
+                V8<double> v = zero8<double, MASK>();
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) v.c[c] = qd[V::slot(c)][j];
+                return v;
+            };
+            auto CQ = [&](int j) {
+                V8<double> v = zero8<double, MASK>();
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) v.c[c] = cq[V::slot(c)][j];
+                return v;
+            };
+            const V8<double> z8 = zero8<double, MASK>();
+            w = ho_cell_prim<MASK, 1>(Q(iq), Q(iq - 1), Q(iq + 1), z8, z8, CQ(iq), CQ(iq - 1), CQ(iq + 1), z8, z8, gamma);
+        } else {
+            V8<double> q = zero8<double, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) q.c[c] = qd[V::slot(c)][iq];
+            w = cons_to_prim<double, MASK>(q, gamma);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) ws[V::slot(c)][i] = R(w.c[c]);
+    }
+    __syncthreads();
+    struct WAcc {
+        R (*ws)[NW];
+        int bw, N, bc;
+        __device__ int e(int p) const { return bc == EDGE ? clampi(p, 0, N - 1) : p; }
+        __device__ V8<R> operator()(int m) const {
+            const int j = e(m) - bw;
+            V8<R> v = zero8<R, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (VS<MASK>::live(c)) v.c[c] = ws[VS<MASK>::slot(c)][j];
+            return v;
+        }
+    } W{ws, bw, N, bc};
+    // C: reconstruction at cells [s0-1, s0+TS+2)
+    if constexpr (RECON != PCM) {
+        for (int i = threadIdx.x; i < NC; i += blockDim.x) {
+            V8<R> L, Rr;
+            recon_cell<R, MASK, RECON, 0>(W, e(bcl + i), a.limiter, a.flattener, L, Rr);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) {
+                    wl[V::slot(c)][i] = L.c[c];
+                    wr[V::slot(c)][i] = Rr.c[c];
+                }
+        }
+        __syncthreads();
+    }
+    auto WL = [&](int m) {  // wL at cell m (stored values already follow the rule)
+        if constexpr (RECON == PCM) return W(m);
+        else {
+            V8<R> v = zero8<R, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) v.c[c] = wl[V::slot(c)][m - bcl];
+            return v;
+        }
+    };
+    auto WR = [&](int m) {
+        if constexpr (RECON == PCM) return W(m);
+        else {
+            V8<R> v = zero8<R, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) v.c[c] = wr[V::slot(c)][m - bcl];
+            return v;
+        }
+    };
+    // D: spectral radius of the Jacobian state at padded positions j in [s0, s0+TS+1]
+    double lmax = 0.0;
+    bool bad = false;
+    for (int i = threadIdx.x; i < NR; i += blockDim.x) {
+        const int j = s0 + i;
+        V8<R> st;
+        if constexpr (RECON == PCM) {
+            st = W(e(j - 1));  // pcm.py:28-30: Jacobian at padded cells
+        } else {
+            const int je = bc == EDGE ? clampi(j, 1, N) : j;
+            const V8<R> wp = WL(je), wm = WR(je - 1);
+            st = (RECON == PLM) ? arith_avg<R, MASK>(wp, wm) : roe_avg<R, MASK>(wp, wm);
+        }
+        const R r = spectral_radius<R, MASK, 0>(st, gR);
+        rad[i] = r;
+        if (j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= N) {
+            const double rd = double(r);
+            if (!(rd <= 1.7976931348623157e308)) bad = true;
+            lmax = (rd > lmax || rd != rd) ? rd : lmax;
+        }
+    }
+    if (bad) atomicOr(&ps->fail_flag, 1);
+    if (a.stage0) block_max_atomic(lmax, &ps->eig[0], red);
+    __syncthreads();
+    // E: interface fluxes k in [s0, s0+TS]
+    for (int i = threadIdx.x; i < NF; i += blockDim.x) {
+        const int k = s0 + i;
+        Iface<R> f;
+        f.wm = WR(k - 1);
+        f.wp = WL(k);
+        if constexpr (RECON == PCM) {
+            auto Qc = [&](int m) {
+                V8<double> v = zero8<double, MASK>();
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) v.c[c] = qd[V::slot(c)][e(m) - bq];
+                return cast8<R, double, MASK>(v);
+            };
+            f.qm = Qc(k - 1);
+            f.qp = Qc(k);
+            f.wm = W(e(k - 1));
+            f.wp = W(e(k));
+        } else {
+            f.qm = prim_to_cons<R, MASK>(f.wm, gR);
+            f.qp = prim_to_cons<R, MASK>(f.wp, gR);
+        }
+        f.fm = phys_flux<R, MASK, 0>(f.wm, gR);
+        f.fp = phys_flux<R, MASK, 0>(f.wp, gR);
+        const R lam = nmax(rad[i], rad[i + 1]);
+        R Bn = R(0);
+        if constexpr (SOLVER == HLLD) Bn = W(e(k)).c[5];
+        const V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) fl[V::slot(c)][i] = F.c[c];
+        if (a.flux_out && k <= N && (k < s0 + TS || k == N)) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) a.flux_out[V::slot(c) * a.flux_plane + k] = double(F.c[c]);
+        }
+    }
+    __syncthreads();
+    // F: L = -(F[i+1] - F[i]) / dx (evolvers.py:41-60)
+    const R dxR = R(pd.dx);
+    for (int i = threadIdx.x; i < TS; i += blockDim.x) {
+        const int cell = s0 + i;
+        if (cell >= N) break;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const R d = (fl[v][i + 1] - fl[v][i]) / dxR;
+            a.out[v * P + off + cell] = -double(d);
+        }
+    }
+}
+
+// ============================================================================
+// 2D reconstruction kernel: stage input -> wS (4th-order or point conversion)
+// -> limited interface states wL/wR of the tile's own cells, written in the
+// sweep-frame layout [t][s] (s contiguous) of type R.
+// ============================================================================
+template <class R, int MASK, int RECON, int AX, int TS, int TT>
+__global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
+    using V = VS<MASK>;
+    constexpr int NV = V::NV;
+    constexpr bool HO = (RECON == PPM || RECON >= WENO3);
+    constexpr int HS = 4;                    // wS window [s0-4, s0+TS+4)
+    constexpr int NWS = TS + 2 * HS;
+    constexpr int NQS = NWS + 2;             // q window  [s0-5, s0+TS+5)
+    constexpr int HT = HO ? 1 : 0;           // q rows    [t0-HT, t0+TT+HT)
+    constexpr int NQT = TT + 2 * HT;
+    __shared__ double qd[NV][NQT][NQS];
+    __shared__ double cq[HO ? NV : 1][HO ? NQT : 1][HO ? NQS : 1];
+    __shared__ R ws[NV][TT][NWS];
+
+    const Tile tile = a.tiles[blockIdx.x];
+    const ProbDesc pd = a.pd[tile.prob];
+    ProbState* ps = a.ps + tile.prob;
+    if (!ps->active) return;
+    const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
+    const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
+    const double gamma = pd.gamma;
+    const long long off = pd.off, P = a.plane;
+    auto gidx = [&](int s, int t) { return AX == 0 ? s * pd.n1 + t : t * pd.n1 + s; };
+    auto es = [&](int p) { return bc == EDGE ? clampi(p, 0, ns - 1) : p; };
+    auto et = [&](int p) { return bc == EDGE ? clampi(p, 0, nt - 1) : p; };
+    const int bqs = s0 - HS - 1, bqt = t0 - HT, bws = s0 - HS;
+
+    // A: load, memory-contiguous index fastest
+    for (int i = threadIdx.x; i < NQT * NQS; i += blockDim.x) {
+        int is, it;
+        if constexpr (AX == 0) { is = i / NQT; it = i % NQT; }
+        else { it = i / NQS; is = i % NQS; }
+        const int gs = bcmap(bqs + is, ns, bc), gt = bcmap(bqt + it, nt, bc);
+        const long long g = off + gidx(gs, gt);
+        V8<double> q = zero8<double, MASK>();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) q.c[c] = a.X[V::slot(c) * P + g];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) qd[V::slot(c)][it][is] = q.c[c];
+        if constexpr (HO) {
+            const V8<double> w = cons_to_prim<double, MASK>(q, gamma);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) cq[V::slot(c)][it][is] = w.c[c];
+        }
+    }
+    __syncthreads();
+    // B: wS for s in [s0-4, s0+TS+4), own rows t in [t0, t0+TT)
+    for (int i = threadIdx.x; i < TT * NWS; i += blockDim.x) {
+        const int it = i / NWS, is = i % NWS;
+        const int t = t0 + it;
+        const int js = es(bws + is) - bqs;
+        const int jt = et(t) - bqt;
+        auto Q = [&](int x, int y) {
+            V8<double> v = zero8<double, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) v.c[c] = qd[V::slot(c)][y][x];
+            return v;
+        };
+        V8<double> w;
+        if constexpr (HO) {
+            auto CQ = [&](int x, int y) {
+                V8<double> v = zero8<double, MASK>();
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) v.c[c] = cq[V::slot(c)][y][x];
+                return v;
+            };
+            w = ho_cell_prim<MASK, 2>(Q(js, jt), Q(js - 1, jt), Q(js + 1, jt), Q(js, jt - 1), Q(js, jt + 1),
+                                      CQ(js, jt), CQ(js - 1, jt), CQ(js + 1, jt), CQ(js, jt - 1), CQ(js, jt + 1),
+                                      gamma);
+        } else {
+            w = cons_to_prim<double, MASK>(Q(js, jt), gamma);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) ws[V::slot(c)][it][is] = R(w.c[c]);
+    }
+    __syncthreads();
+    // C: reconstruct own cells, write wL/wR (sweep frame [t][s])
+    R* wLo = reinterpret_cast<R*>(a.wL);
+    R* wRo = reinterpret_cast<R*>(a.wR);
+    const long long fo = off;  // same packing as the state, sweep-frame order
+    for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
+        const int it = i / TS, is = i % TS;
+        const int s = s0 + is, t = t0 + it;
+        if (s >= ns || t >= nt) continue;
+        struct WAcc {
+            R (*row)[TT][NWS];
+            int it, bws, n, bc;
+            __device__ int e(int p) const { return bc == EDGE ? clampi(p, 0, n - 1) : p; }
+            __device__ V8<R> operator()(int m) const {
+                const int j = e(m) - bws;
+                V8<R> v = zero8<R, MASK>();
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (VS<MASK>::live(c)) v.c[c] = row[VS<MASK>::slot(c)][it][j];
+                return v;
+            }
+        } W{ws, it, bws, ns, bc};
+        V8<R> L, Rr;
+        recon_cell<R, MASK, RECON, AX>(W, s, a.limiter, a.flattener, L, Rr);
+        const long long o = fo + (long long)t * ns + s;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) {
+                wLo[V::slot(c) * P + o] = L.c[c];
+                wRo[V::slot(c) * P + o] = Rr.c[c];
+            }
+        if constexpr (V::MHD) {
+            // HLLD's normal field: component (solver axis + 5) of the centred
+            // state, the solver axis being the sweep's enumeration index
+            // (solvers.py:160,168,183)
+            if (a.bn) {
+                const int parity = a.parity_override >= 0 ? a.parity_override : ps->parity;
+                const int sax = parity == 0 ? AX : 1 - AX;
+                const V8<R> z = W(s);
+                reinterpret_cast<R*>(a.bn)[o] = sax == 0 ? z.c[5] : z.c[6];
+            }
+        }
+    }
+}
+
+// ============================================================================
+// 2D flux kernel: interface states (sweep frame) -> face conversions ->
+// Riemann on face averages -> second Riemann on face-centred states ->
+// transverse correction (functions/fv.py:147-153) -> flux difference.
+// CONV: 0 = PCM (q from the cell state), 1 = point (PLM), 2 = 4th-order face
+// conversion (PPM/WENO, fv.py:105-122 with 'face').
+// ============================================================================
+template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT>
+__global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
+    using V = VS<MASK>;
+    constexpr int NV = V::NV;
+    constexpr int NK = TS + 2;   // interfaces [s0, s0+TS+1]
+    constexpr int NT = TT + 4;   // w lines   [t0-2, t0+TT+2)
+    constexpr int NKF = TS + 1;  // flux interfaces [s0, s0+TS]
+    constexpr int NTA = TT + 2;  // face-average lines [t0-1, t0+TT+1)
+    __shared__ R wm_s[NV][NT][NK];
+    __shared__ R wp_s[NV][NT][NK];
+    __shared__ R qm_s[NV][NTA][NKF];
+    __shared__ R qp_s[NV][NTA][NKF];
+    __shared__ R fm_s[NV][NTA][NKF];
+    __shared__ R fp_s[NV][NTA][NKF];
+    __shared__ R fa_s[NV][NTA][NKF];
+    __shared__ R rad_s[NTA][NK];
+    __shared__ R bn_s[SOLVER == HLLD ? NTA : 1][SOLVER == HLLD ? NKF : 1];
+    __shared__ R F_s[NV][TT][NKF];
+    __shared__ double red[4];
+
+    const Tile tile = a.tiles[blockIdx.x];
+    const ProbDesc pd = a.pd[tile.prob];
+    ProbState* ps = a.ps + tile.prob;
+    if (!ps->active) return;
+    const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
+    const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
+    const R gR = R(pd.gamma);
+    const long long off = pd.off, P = a.plane;
+    const int parity = a.parity_override >= 0 ? a.parity_override : ps->parity;
+    const int sax = parity == 0 ? AX : 1 - AX;  // enumeration index (solvers.py:34-36)
+    auto et = [&](int p) { return bc == EDGE ? clampi(p, 0, nt - 1) : p; };
+    const int bt = t0 - 2, bta = t0 - 1;
+    const R* wLg = reinterpret_cast<const R*>(a.wL);
+    const R* wRg = reinterpret_cast<const R*>(a.wR);
+    auto sidx = [&](int s, int t) { return off + (long long)t * ns + s; };  // sweep-frame layout
+    auto gidx = [&](int s, int t) { return off + (AX == 0 ? (long long)s * pd.n1 + t : (long long)t * pd.n1 + s); };
+
+    // A: w-/w+ at interfaces [s0, s0+TS+1] x lines [t0-2, t0+TT+2)
+    for (int i = threadIdx.x; i < NT * NK; i += blockDim.x) {
+        const int it = i / NK, ik = i % NK;
+        const int k = s0 + ik;
+        const int gt = bcmap(bt + it, nt, bc);
+        const long long om = sidx(bcmap(k - 1, ns, bc), gt), op = sidx(bcmap(k, ns, bc), gt);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            wm_s[v][it][ik] = wRg[v * P + om];
+            wp_s[v][it][ik] = wLg[v * P + op];
+        }
+    }
+    __syncthreads();
+    auto WM = [&](int it, int ik) {
+        V8<R> w = zero8<R, MASK>();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) w.c[c] = wm_s[V::slot(c)][it][ik];
+        return w;
+    };
+    auto WP = [&](int it, int ik) {
+        V8<R> w = zero8<R, MASK>();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) w.c[c] = wp_s[V::slot(c)][it][ik];
+        return w;
+    };
+    // B: spectral radius at padded Jacobian positions j in [s0, s0+TS+1],
+    // lines [t0-1, t0+TT+1) (evaluated on the boundary-mapped line)
+    double lmax = 0.0;
+    bool bad = false;
+    for (int i = threadIdx.x; i < NTA * NK; i += blockDim.x) {
+        const int ia = i / NK, ik = i % NK;
+        const int t = bta + ia, te = et(t), it = te - bt;
+        const int j = s0 + ik;
+        V8<R> st;
+        if constexpr (CONV == 0) {
+            // PCM: Jacobian at cell j-1 = w_minus at interface j
+            st = WM(it, ik);
+        } else {
+            const int je = bc == EDGE ? clampi(j, 1, ns) : j;
+            const int jk = je - s0;  // in [0, NK) or the clamped edge interface
+            const V8<R> wp = WP(it, jk), wm = WM(it, jk);
+            st = (CONV == 1) ? arith_avg<R, MASK>(wp, wm) : roe_avg<R, MASK>(wp, wm);
+        }
+        const R r = spectral_radius<R, MASK, AX>(st, gR);
+        rad_s[ia][ik] = r;
+        if (t >= t0 && t < t0 + TT && t < nt && j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= ns) {
+            const double rd = double(r);
+            if (!(rd <= 1.7976931348623157e308)) bad = true;
+            lmax = (rd > lmax || rd != rd) ? rd : lmax;
+        }
+    }
+    if (bad) atomicOr(&ps->fail_flag, 1);
+    if (a.stage0) block_max_atomic(lmax, &ps->eig[AX], red);
+    __syncthreads();
+    // C: q+-, f+- and the face-average flux on lines [t0-1, t0+TT+1)
+    const double* Xg = a.X;
+    for (int i = threadIdx.x; i < NTA * NKF; i += blockDim.x) {
+        const int ia = i / NKF, ik = i % NKF;
+        const int t = bta + ia, te = et(t), it = te - bt;
+        const int k = s0 + ik;
+        Iface<R> f;
+        f.wm = WM(it, ik);
+        f.wp = WP(it, ik);
+        if constexpr (CONV == 0) {
+            const int gt = bcmap(te, nt, bc);
+            V8<double> qm = zero8<double, MASK>(), qp = zero8<double, MASK>();
+            const long long om = gidx(bcmap(k - 1, ns, bc), gt), op = gidx(bcmap(k, ns, bc), gt);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) {
+                    qm.c[c] = Xg[V::slot(c) * P + om];
+                    qp.c[c] = Xg[V::slot(c) * P + op];
+                }
+            f.qm = cast8<R, double, MASK>(qm);
+            f.qp = cast8<R, double, MASK>(qp);
+        } else if constexpr (CONV == 1) {
+            f.qm = prim_to_cons<R, MASK>(f.wm, gR);
+            f.qp = prim_to_cons<R, MASK>(f.wp, gR);
+        } else {
+            // face conversion along t (fv.py:105-122, axes = [1] of the sweep frame):
+            // q = P(w - K24 lap_t w) + K24 lap_t P(w)
+            auto face = [&](const V8<R>& wc, const V8<R>& wlo, const V8<R>& whi) {
+                const V8<R> pc = prim_to_cons<R, MASK>(wc, gR), plo = prim_to_cons<R, MASK>(wlo, gR),
+                            phi = prim_to_cons<R, MASK>(whi, gR);
+                V8<R> ww = wc;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) ww.c[c] = wc.c[c] - R(K24) * ((whi.c[c] - wc.c[c]) - (wc.c[c] - wlo.c[c]));
+                V8<R> q = prim_to_cons<R, MASK>(ww, gR);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) q.c[c] = q.c[c] + R(K24) * ((phi.c[c] - pc.c[c]) - (pc.c[c] - plo.c[c]));
+                return q;
+            };
+            const int itm = et(te - 1) - bt, itp = et(te + 1) - bt;
+            f.qm = face(f.wm, WM(itm, ik), WM(itp, ik));
+            f.qp = face(f.wp, WP(itm, ik), WP(itp, ik));
+        }
+        f.fm = phys_flux<R, MASK, AX>(f.wm, gR);
+        f.fp = phys_flux<R, MASK, AX>(f.wp, gR);
+        R Bn = R(0);
+        if constexpr (SOLVER == HLLD) {
+            const int gt = bcmap(te, nt, bc);
+            Bn = reinterpret_cast<const R*>(a.bn)[sidx(bcmap(k, ns, bc), gt)];
+            bn_s[ia][ik] = Bn;
+        }
+        const R lam = nmax(rad_s[ia][ik], rad_s[ia][ik + 1]);
+        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) {
+                const int v = V::slot(c);
+                qm_s[v][ia][ik] = f.qm.c[c];
+                qp_s[v][ia][ik] = f.qp.c[c];
+                fm_s[v][ia][ik] = f.fm.c[c];
+                fp_s[v][ia][ik] = f.fp.c[c];
+                fa_s[v][ia][ik] = F.c[c];
+            }
+    }
+    __syncthreads();
+    // D: centred second solve + transverse correction on own lines
+    for (int i = threadIdx.x; i < TT * NKF; i += blockDim.x) {
+        const int itt = i / NKF, ik = i % NKF;
+        const int t = t0 + itt;
+        const int te = et(t);
+        const int ia0 = te - bta, iam = et(te - 1) - bta, iap = et(te + 1) - bta;
+        const int it0 = te - bt, itm = et(te - 1) - bt, itp = et(te + 1) - bt;
+        auto corr = [&](R (*arr)[NT][NK], int v) {
+            const R x0 = arr[v][it0][ik], xm = arr[v][itm][ik], xp = arr[v][itp][ik];
+            return x0 - R(K24) * ((xp - x0) - (x0 - xm));
+        };
+        auto corrA = [&](R (*arr)[NTA][NKF], int v) {
+            const R x0 = arr[v][ia0][ik], xm = arr[v][iam][ik], xp = arr[v][iap][ik];
+            return x0 - R(K24) * ((xp - x0) - (x0 - xm));
+        };
+        Iface<R> f;
+        f.wm = zero8<R, MASK>();
+        f.wp = f.qm = f.qp = f.fm = f.fp = f.wm;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) {
+                const int v = V::slot(c);
+                f.wm.c[c] = corr(wm_s, v);
+                f.wp.c[c] = corr(wp_s, v);
+                f.qm.c[c] = corrA(qm_s, v);
+                f.qp.c[c] = corrA(qp_s, v);
+                f.fm.c[c] = corrA(fm_s, v);
+                f.fp.c[c] = corrA(fp_s, v);
+            }
+        const R lam = nmax(rad_s[ia0][ik], rad_s[ia0][ik + 1]);
+        R Bn = R(0);
+        if constexpr (SOLVER == HLLD) Bn = bn_s[ia0][ik];
+        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR);
+        const int k = s0 + ik;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (V::live(c)) {
+                const int v = V::slot(c);
+                const R F = Fc.c[c] - R(K24) * ((fa_s[v][iap][ik] - fa_s[v][ia0][ik]) - (fa_s[v][ia0][ik] - fa_s[v][iam][ik]));
+                F_s[v][itt][ik] = F;
+                if (a.flux_out && t < nt && k <= ns && (k < s0 + TS || k == ns))
+                    a.flux_out[v * a.flux_plane + (long long)k * nt + t] = double(F);
+            }
+    }
+    __syncthreads();
+    // E: flux difference; first sweep writes D, second writes L = -(D0 + D1)
+    const R dxR = R(pd.dx);
+    for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
+        int itt, is;
+        if constexpr (AX == 0) { is = i / TT; itt = i % TT; }
+        else { itt = i / TS; is = i % TS; }
+        const int s = s0 + is, t = t0 + itt;
+        if (s >= ns || t >= nt) continue;
+        const long long g = gidx(s, t);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double d = double((F_s[v][itt][is + 1] - F_s[v][itt][is]) / dxR);
+            if (a.last)
+                a.out[v * P + g] = -(a.Dprev[v * P + g] + d);
+            else
+                a.out[v * P + g] = d;
+        }
+    }
+}
+
+}  // namespace ah

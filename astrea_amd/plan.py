@@ -1,0 +1,183 @@
+"""Host side of the device update: a Plan owns one ah_plan, its packed state
+and its workspace as torch CUDA tensors, and moves states between the
+reference layout (cells..., 8 conservative components) and the packed SoA
+layout of the library (include/astrea_hip.h)."""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import options as O
+
+BOUNDARY = {"edge": L.AH_EDGE, "outflow": L.AH_EDGE, "wrap": L.AH_WRAP, "periodic": L.AH_WRAP}
+
+
+def infer_varmask(grids, dims, magnetic):
+    """Smallest instantiated component set that holds every nonzero component
+    of the initial states.  Components that are zero everywhere stay zero
+    under the update for the classes named here (no transverse velocity or
+    field source in 1D/2D hydro; v_z = B_z = 0 is preserved in 2D MHD)."""
+    nz = np.zeros(8, bool)
+    for g in grids:
+        nz |= np.any(np.asarray(g).reshape(-1, 8) != 0, axis=0)
+    if dims == 1:
+        if not magnetic and not nz[[2, 3, 5, 6, 7]].any():
+            return L.AH_VARS_H1
+        return L.AH_VARS_ALL
+    if not magnetic and not nz[[3, 5, 6, 7]].any():
+        return L.AH_VARS_H2
+    if not nz[[3, 7]].any():
+        return L.AH_VARS_M6
+    raise NotImplementedError("2D states with nonzero v_z or B_z have no device path")
+
+
+class Plan:
+    def __init__(self, dims, subgrid, solver, timestep, problems, *, limiter="minmod", flattener="off",
+                 precision="fp64", varmask=L.AH_VARS_H1, device="cuda:0"):
+        self.lib = L.load()
+        self.dims = dims
+        self.device = torch.device(device)
+        sch = L.AhScheme()
+        sch.dims = dims
+        sch.recon = O.recon_of(subgrid)
+        sch.limiter = O.LIMITERS[limiter]
+        sch.flattener = O.FLATTENERS[flattener]
+        sch.solver = O.solver_of(solver)
+        sch.rk = O.rk_of(timestep)
+        sch.precision = L.AH_SPLIT if precision == "split" else L.AH_FP64
+        sch.varmask = varmask
+        sch.magnetic_2d = 0
+        self.scheme = sch
+        arr = (L.AhProblem * len(problems))()
+        for i, p in enumerate(problems):
+            arr[i].n0 = int(p["n0"])
+            arr[i].n1 = int(p.get("n1", 1)) if dims == 2 else 1
+            arr[i].boundary = BOUNDARY[p.get("boundary", "edge")]
+            arr[i].checkpoints = int(p.get("checkpoints", 1))
+            arr[i].dx = float(p["dx"])
+            arr[i].gamma = float(p["gamma"])
+            arr[i].cfl = float(p["cfl"])
+            arr[i].t_end = float(p["t_end"])
+            arr[i].max_steps = int(p.get("max_steps", -1))
+        self.problems = arr
+        self.nprob = len(problems)
+        h = ctypes.c_void_p()
+        L.check(self.lib.ah_plan_create(ctypes.byref(sch), self.nprob, arr, ctypes.byref(h)), "ah_plan_create")
+        self.h = h
+        total, nv = ctypes.c_longlong(), ctypes.c_int()
+        vidx = (ctypes.c_int * 8)()
+        offs = (ctypes.c_longlong * self.nprob)()
+        L.check(self.lib.ah_plan_layout(h, ctypes.byref(total), ctypes.byref(nv), vidx, offs), "ah_plan_layout")
+        self.total, self.nv = total.value, nv.value
+        self.varidx = [vidx[i] for i in range(self.nv)]
+        self.offsets = [offs[i] for i in range(self.nprob)]
+        ws = ctypes.c_size_t()
+        L.check(self.lib.ah_plan_workspace_size(h, ctypes.byref(ws)), "ah_plan_workspace_size")
+        self.state = torch.zeros((self.nv, self.total), dtype=torch.float64, device=self.device)
+        self.workspace = torch.empty(max(ws.value, 1), dtype=torch.uint8, device=self.device)
+        L.check(self.lib.ah_plan_bind(h, ctypes.c_void_p(self.state.data_ptr()),
+                                      ctypes.c_void_p(self.workspace.data_ptr()), ws.value), "ah_plan_bind")
+
+    # ---- helpers -----------------------------------------------------------
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def shape(self, i):
+        p = self.problems[i]
+        return (p.n0,) if self.dims == 1 else (p.n0, p.n1)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ah_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state in/out --------------------------------------------------------
+    def load(self, grids):
+        """Pack conservative states given in the reference layout."""
+        host = np.zeros((self.nv, self.total))
+        for i, g in enumerate(grids):
+            g = np.asarray(g, dtype=np.float64)
+            assert g.shape == self.shape(i) + (8,), (g.shape, self.shape(i))
+            flat = g.reshape(-1, 8)
+            o = self.offsets[i]
+            for v, c in enumerate(self.varidx):
+                host[v, o:o + flat.shape[0]] = flat[:, c]
+            dead = [c for c in range(8) if c not in self.varidx]
+            if dead and np.any(flat[:, dead] != 0):
+                raise ValueError(f"problem {i} has nonzero components {dead} outside the plan's variable set")
+        self.state.copy_(torch.from_numpy(host).to(self.device))
+
+    def unpack(self, i=None, state=None):
+        """States of one or all problems in the reference layout."""
+        host = (self.state if state is None else state).detach().cpu().numpy()
+        out = []
+        for k in range(self.nprob) if i is None else [i]:
+            n = int(np.prod(self.shape(k)))
+            g = np.zeros((n, 8))
+            o = self.offsets[k]
+            for v, c in enumerate(self.varidx):
+                g[:, c] = host[v, o:o + n]
+            out.append(g.reshape(self.shape(k) + (8,)))
+        return out if i is None else out[0]
+
+    # ---- time stepping -------------------------------------------------------
+    def reset(self):
+        L.check(self.lib.ah_plan_reset(self.h, self._stream()), "ah_plan_reset")
+
+    def step(self, nsteps=1):
+        L.check(self.lib.ah_plan_step(self.h, int(nsteps), self._stream()), "ah_plan_step")
+
+    def status(self):
+        st = (L.AhProblemStatus * self.nprob)()
+        L.check(self.lib.ah_plan_status(self.h, st, self._stream()), "ah_plan_status")
+        return [dict(t=s.t, dt=s.dt, steps=s.steps, active=bool(s.active), failed=bool(s.failed)) for s in st]
+
+    def run(self, chunk=32, max_chunks=10 ** 9):
+        """Step until no problem is active; the host looks at the device time
+        state once per chunk of steps."""
+        for _ in range(max_chunks):
+            self.step(chunk)
+            st = self.status()
+            if not any(s["active"] for s in st):
+                return st
+        return self.status()
+
+    # ---- parity entry points (single-problem plans) ----------------------------
+    def evolve_space(self, parity=0, state=None):
+        """Interface fluxes per physical axis in the reference's sweep-frame
+        layout ((n_a + 1, n_other, 8) in 2D) and eigmax per axis."""
+        assert self.nprob == 1
+        p = self.problems[0]
+        X = self.state if state is None else state
+        n0, n1 = p.n0, p.n1
+        f0 = torch.zeros((self.nv, (n0 + 1) * n1), dtype=torch.float64, device=self.device)
+        f1 = torch.zeros((self.nv, (n1 + 1) * n0), dtype=torch.float64, device=self.device)
+        eig = (ctypes.c_double * 2)()
+        L.check(self.lib.ah_evolve_space(self.h, 0, int(parity), ctypes.c_void_p(X.data_ptr()),
+                                         ctypes.c_void_p(f0.data_ptr()), ctypes.c_void_p(f1.data_ptr()), eig,
+                                         self._stream()), "ah_evolve_space")
+        out = {}
+        for ax, (f, ns, nt) in enumerate([(f0, n0, n1), (f1, n1, n0)][:self.dims]):
+            h = f.cpu().numpy()
+            g = np.zeros(((ns + 1) * nt, 8))
+            for v, c in enumerate(self.varidx):
+                g[:, c] = h[v]
+            shape = (ns + 1,) if self.dims == 1 else (ns + 1, nt)
+            out[ax] = {"flux": g.reshape(shape + (8,)), "eigmax": eig[ax], "raw": f}
+        return out
+
+    def compute_L(self, fluxes, parity=0):
+        L_ = torch.zeros_like(self.state)
+        f0 = fluxes[0]["raw"]
+        f1 = fluxes[1]["raw"] if self.dims == 2 else f0
+        L.check(self.lib.ah_compute_L(self.h, 0, int(parity), ctypes.c_void_p(f0.data_ptr()),
+                                      ctypes.c_void_p(f1.data_ptr()), ctypes.c_void_p(L_.data_ptr()),
+                                      self._stream()), "ah_compute_L")
+        return L_

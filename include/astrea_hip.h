@@ -1,0 +1,155 @@
+/*
+ * astrea_hip.h -- C ABI of libastrea_hip.so, the MI355X (gfx950) finite-volume
+ * update for astrea (mervyzr/astrea 1.3.2).
+ *
+ * The reference has no FFI: its operator interface for this path is the pair
+ *   evolve_space(grid, sim_variables) -> fluxes   (num_methods/evolvers.py:12)
+ *   evolve_time(grid, fluxes, dt, sim_variables) -> grid (num_methods/evolvers.py:38)
+ * driven by core_run's time loop (astrea.py:31-85).  The entry points below
+ * replace those calls; the comment on each names the reference code it stands
+ * in for.  Only plain C types cross the ABI; device pointers are HIP device
+ * addresses (e.g. torch CUDA-tensor data pointers), ``stream`` is a hipStream_t
+ * passed as void*.
+ *
+ * State layout ("packed SoA"): for a plan over P problems with C_p cells each
+ * (C_p = n0*n1, cell (i0, i1) at i0*n1 + i1, i.e. the reference's C order of
+ * grid[i0, i1]), the state is nvars planes of sum(C_p) doubles; plane v holds
+ * conservative component varidx[v] of every problem back to back.  The
+ * components of a plan are the live ones of its variable mask (bit c set =
+ * component c of rho, rho*vx, rho*vy, rho*vz, E, Bx, By, Bz is carried);
+ * components outside the mask are identically zero in the reference for the
+ * problem class the mask names and are not stored.
+ *
+ * All functions return AH_OK (0) or a negative status; none throws.
+ * A plan is single-stream and not re-entrant; distinct plans are independent.
+ */
+#ifndef ASTREA_HIP_H
+#define ASTREA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AH_VERSION_MAJOR 1
+#define AH_VERSION_MINOR 0
+
+enum ah_status {
+    AH_OK = 0,
+    AH_ERR_INVALID = -1,     /* bad argument or option combination */
+    AH_ERR_UNSUPPORTED = -2, /* option the device path does not implement */
+    AH_ERR_HIP = -3,         /* a HIP runtime call failed */
+    AH_ERR_NOMEM = -4,       /* workspace too small / allocation failed */
+    AH_ERR_STATE = -5        /* plan not bound, or used out of order */
+};
+
+/* options -- same meaning as astrea's option strings (static/.db.json:220-347) */
+enum ah_boundary { AH_EDGE = 0, AH_WRAP = 1 };                /* "edge" (outlet) / "wrap" (periodic) */
+enum ah_recon { AH_PCM = 0, AH_PLM = 1, AH_PPM = 2, AH_WENO3 = 3, AH_WENO5 = 4, AH_WENO7 = 5 };
+enum ah_limiter { AH_MINMOD = 0, AH_VANLEER = 1, AH_OSPRE = 2, AH_VANALBADA1 = 3, AH_KOREN = 4, AH_SUPERBEE = 5 };
+enum ah_flattener { AH_FLAT_OFF = 0, AH_FLAT_ASTREA = 1, AH_FLAT_COLELLA90 = 2 };
+enum ah_solver { AH_LLF = 0, AH_HLLC = 1, AH_HLLD = 2 };
+enum ah_rk {
+    AH_EULER = 0, AH_RK4 = 1, AH_SSPRK22 = 2, AH_SSPRK33 = 3, AH_SSPRK43 = 4,
+    AH_SSPRK53 = 5, AH_SSPRK54 = 6, AH_SSPRK104 = 7
+};
+enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };  /* SPLIT: FP32 reconstruction/limiter/Riemann, FP64 state */
+enum ah_varmask {
+    AH_VARS_H1 = 0x13,  /* rho, rho*vx, E          : 1D hydrodynamics */
+    AH_VARS_H2 = 0x17,  /* rho, rho*vx, rho*vy, E  : 2D hydrodynamics */
+    AH_VARS_M6 = 0x77,  /* + Bx, By                : 2D MHD with vz = Bz = 0 */
+    AH_VARS_ALL = 0xFF
+};
+
+typedef struct ah_scheme {
+    int dims;        /* 1 or 2 (sim_variables.dimension) */
+    int recon;       /* ah_recon   (subgrid, evolvers.py:14-21) */
+    int limiter;     /* ah_limiter (PLM slope limiter; reference: minmod, plm.py:66) */
+    int flattener;   /* ah_flattener (PPM only) */
+    int solver;      /* ah_solver  (solvers.py:13-31) */
+    int rk;          /* ah_rk      (evolvers.py:79-206) */
+    int precision;   /* ah_precision */
+    int varmask;     /* ah_varmask */
+    int magnetic_2d; /* constrained transport (evolvers.py:26-32, 51-58, 63-67, 73-76) */
+    int tile_s;      /* sweep tile length in cells (0: default / autotuned) */
+    int tile_t;      /* transverse tile width in lines, 2D only (0: default) */
+    int reserved[5];
+} ah_scheme;
+
+typedef struct ah_problem {
+    int n0, n1;       /* cells on axis 0 and axis 1 (n1 = 1 in 1D) */
+    int boundary;     /* ah_boundary */
+    int checkpoints;  /* sim_variables.checkpoints (astrea.py:32) */
+    double dx;        /* sim_variables.dx (used for both axes, evolvers.py:48) */
+    double gamma;
+    double cfl;
+    double t_end;
+    long long max_steps; /* < 0: no step limit (the reference runs to t_end only) */
+} ah_problem;
+
+typedef struct ah_problem_status {
+    double t;          /* simulated time (astrea.py:84) */
+    double dt;         /* last dt taken */
+    long long steps;   /* steps taken */
+    int active;        /* 1 while t < t_end and steps < max_steps */
+    int failed;        /* 1 once a non-finite Jacobian state appeared (the reference's LAPACK call raises there) */
+} ah_problem_status;
+
+typedef struct ah_plan ah_plan;
+
+/* Library identity. */
+int ah_version(int* major, int* minor);
+const char* ah_status_str(int status);
+
+/* Build a plan for `nprob` independent problems sharing one scheme (one problem
+ * = the reference's single run; several = the ragged batch).  Allocates the
+ * plan's small device tables (problem descriptors, tile lists, time state). */
+int ah_plan_create(const ah_scheme* scheme, int nprob, const ah_problem* problems, ah_plan** out);
+int ah_plan_destroy(ah_plan* plan);
+
+/* Packed state layout: total cells, number of stored components and their
+ * component indices (varidx[v] in 0..7), first cell of each problem. */
+int ah_plan_layout(const ah_plan* plan, long long* total_cells, int* nvars, int* varidx,
+                   long long* problem_offsets);
+
+/* Caller-provided device workspace (stage registers, interface states,
+ * reductions).  No allocation happens in ah_plan_step. */
+int ah_plan_workspace_size(const ah_plan* plan, size_t* bytes);
+int ah_plan_bind(ah_plan* plan, double* state, void* workspace, size_t workspace_bytes);
+
+/* t = 0, step counters 0, every problem active (astrea.py:44). */
+int ah_plan_reset(ah_plan* plan, void* stream);
+
+/* Advance every active problem by up to `nsteps` full time steps of the
+ * reference loop (astrea.py:45-85): stage-0 evolve_space, dt = cfl*min(dx/eigmax)
+ * with the checkpoint clamp, evolve_time, t += dt, sweep-order flip.  Time,
+ * dt and termination stay on the device; a problem that reaches t_end or
+ * max_steps (or fails) is skipped by the remaining launches. */
+int ah_plan_step(ah_plan* plan, int nsteps, void* stream);
+
+/* Copy the per-problem time state to the host (synchronises the stream). */
+int ah_plan_status(ah_plan* plan, ah_problem_status* out, void* stream);
+
+/* Parity entry points mirroring the reference's operator interface, for one
+ * problem (index `prob`) of a bound plan, at step parity `parity` (0: axis-0
+ * sweep first, as on even steps; 1: reversed, astrea.py:85).
+ *
+ * ah_evolve_space replaces evolvers.evolve_space + solvers.calculate_Riemann_flux:
+ * writes, for each physical axis a, the interface fluxes in the sweep frame
+ * (flux[a]: nvars planes of (n_a + 1) * n_other doubles, interface-major) and
+ * eigmax[a] (functions/fv.py:157-169, closed-form spectral radius).
+ * `grid` is the packed state of the whole plan (only problem `prob` is read). */
+int ah_evolve_space(ah_plan* plan, int prob, int parity, const double* grid,
+                    double* flux0, double* flux1, double* eigmax, void* stream);
+
+/* ah_compute_L replaces compute_L (evolvers.py:41-60): L = -sum_a dF_a/dx for
+ * problem `prob`, from fluxes laid out as ah_evolve_space writes them. */
+int ah_compute_L(ah_plan* plan, int prob, int parity, const double* flux0, const double* flux1,
+                 double* L, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASTREA_HIP_H */

@@ -1,0 +1,70 @@
+"""CPU checks of the boundary: the in-tree library loads without a GPU and
+exports every entry point include/astrea_hip.h declares; the option surface
+maps astrea's option strings as the reference routes them."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from astrea_amd import _lib as L
+from astrea_amd import options as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "astrea_hip.h")).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(ah_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_matches_binding_list():
+    assert header_functions() == sorted(L.EXPORTS)
+
+
+def test_library_loads_and_exports_every_symbol():
+    if not os.path.exists(L.LIB_PATH):
+        pytest.skip("library not built (run python -m astrea_amd._build)")
+    lib = ctypes.CDLL(L.LIB_PATH)
+    for name in header_functions():
+        assert hasattr(lib, name), name
+    a, b = ctypes.c_int(), ctypes.c_int()
+    assert L.load().ah_version(ctypes.byref(a), ctypes.byref(b)) == 0 and a.value == 1
+    assert L.load().ah_status_str(-2).decode().startswith("option combination")
+
+
+def test_struct_sizes_match_c_layout():
+    assert ctypes.sizeof(L.AhScheme) == 16 * 4
+    assert ctypes.sizeof(L.AhProblem) == 4 * 4 + 4 * 8 + 8
+    assert ctypes.sizeof(L.AhProblemStatus) == 8 * 3 + 2 * 4
+
+
+@pytest.mark.parametrize("s,expect", [("ppm", L.AH_PPM), ("parabolic", L.AH_PPM), ("polynomial", L.AH_PCM),
+                                      ("weno", L.AH_WENO5), ("weno-7", L.AH_WENO7), ("weno 3", L.AH_WENO3),
+                                      ("l", L.AH_PLM), ("c", L.AH_PCM)])
+def test_subgrid_routing(s, expect):
+    # evolvers.py:14-21 (the DB accepts 'polynomial' as ppm but the router sends it to PCM)
+    assert O.recon_of(s) == expect
+
+
+@pytest.mark.parametrize("s,expect", [("ssprk(3,3)", L.AH_SSPRK33), ("ssprk104", L.AH_SSPRK104),
+                                      ("ssprk5,4", L.AH_SSPRK54), ("ssprk(2,2)", L.AH_SSPRK22),
+                                      ("runge-kutta", L.AH_RK4), ("forward euler", L.AH_EULER)])
+def test_timestep_routing(s, expect):
+    assert O.rk_of(s) == expect
+
+
+def test_solver_routing():
+    assert O.solver_of("hllc") == L.AH_HLLC and O.solver_of("c") == L.AH_HLLC
+    assert O.solver_of("d") == L.AH_HLLD
+    assert O.solver_of("lax-wendroff") == L.AH_LLF     # solvers.py:28: only names ending in 'w' reach LW
+    with pytest.raises(NotImplementedError):
+        O.solver_of("lw")
+
+
+def test_invalid_values_revert_to_defaults():
+    sv = O.handle_variables({"config": "khi", "subgrid": "bogus", "solver": "c", "timestep": "ssprk(3,3)",
+                             "cells": 65, "dimension": 1})
+    assert sv.subgrid == "pcm" and sv.dimension == 2 and sv.cells == 64
+    sv = O.handle_variables({"config": "brio-wu", "solver": "hllc"})
+    assert sv.solver == "lf"          # generic.py:266-269

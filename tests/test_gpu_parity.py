@@ -1,0 +1,159 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+reference's golden trajectories.  Every case prints its name before touching
+the GPU."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import astrea_oracle as O
+from tests._golden import load, meta, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+HYDRO = sorted(k for k, v in meta()["cases"].items() if k != "rp1d_fail" and not v.get("magnetic_2d"))
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_run(grid0, m, precision="fp64", chunk=16):
+    from astrea_amd.plan import Plan, infer_varmask
+    dims = m["dimension"]
+    mask = infer_varmask([grid0], dims, m["magnetic_2d"])
+    prob = dict(n0=grid0.shape[0], n1=grid0.shape[1] if dims == 2 else 1, boundary=m["boundary"], dx=m["dx"],
+                gamma=m["gamma"], cfl=m["cfl"], t_end=m["t_end"], checkpoints=m["checkpoints"],
+                max_steps=m["max_steps"] if m["max_steps"] is not None else -1)
+    plan = Plan(dims, m["subgrid"], m["solver"], m["timestep"], [prob], varmask=mask, precision=precision)
+    plan.load([grid0])
+    plan.reset()
+    st = plan.run(chunk)
+    out = plan.unpack(0)
+    plan.close()
+    return out, st[0]
+
+
+def oracle_run(grid0, m, eig="closed"):
+    s = O.Scheme(dimension=m["dimension"], subgrid=m["subgrid"], solver=m["solver"], timestep=m["timestep"],
+                 boundary=m["boundary"], gamma=m["gamma"], cfl=m["cfl"], dx=m["dx"], eig=eig)
+    return O.run(grid0, s, m["t_end"], m["checkpoints"], m["max_steps"])
+
+
+@pytest.mark.parametrize("name", HYDRO)
+def test_trajectory_parity(name):
+    """Same step count as the reference; relative L1 per variable <= 1e-12
+    against the oracle (closed-form eigenvalues, as on the device) and
+    <= 1e-12 against the reference's own trajectory (WENO-7: 1e-10, its
+    weights amplify the ulp-level LAPACK differences)."""
+    m, g = meta()["cases"][name], load("steps.npz")
+    grid0 = g[name + "_grid0"]
+    _say(f"[gpu] trajectory {name} shape={grid0.shape} steps={m['steps']}")
+    out, st = gpu_run(grid0, m)
+    ref = oracle_run(grid0, m)
+    assert not st["failed"]
+    assert st["steps"] == m["steps"] == ref.steps
+    e_or = np.max(rel_l1(out, ref.grid))
+    e_gold = np.max(rel_l1(out, g[name + "_final"]))
+    _say(f"[gpu] {name}: rel L1 vs oracle {e_or:.2e}, vs reference {e_gold:.2e}")
+    assert e_or <= 1e-12
+    assert e_gold <= (1e-10 if "weno7" in name else 1e-12)
+
+
+SCHEME_KEYS = [f"hd_{sg}_{bc}" for sg in ("pcm", "plm", "ppm", "weno3", "weno5", "weno7") for bc in ("edge", "wrap")]
+
+
+@pytest.mark.parametrize("key", SCHEME_KEYS)
+@pytest.mark.parametrize("solver", ["llf", "hllc"])
+def test_interface_flux_1d(key, solver):
+    """ah_evolve_space interface fluxes and eigmax against the reference's
+    solver outputs (schemes_1d.npz) and the oracle."""
+    from astrea_amd.plan import Plan
+    g = load("schemes_1d.npz")
+    _, sg, bc = key.split("_")
+    grid = g[key + "_grid"]
+    N = grid.shape[0]
+    _say(f"[gpu] flux1d {key} {solver}")
+    plan = Plan(1, sg, solver, "euler", [dict(n0=N, boundary=bc, dx=1 / 32, gamma=1.4, cfl=.5, t_end=1.)],
+                varmask=0x13)
+    plan.load([grid])
+    fl = plan.evolve_space(0)
+    plan.close()
+    s = O.Scheme(dimension=1, subgrid=sg, solver=solver, boundary=bc, gamma=1.4, dx=1 / 32)
+    d = O.reconstruct(grid, s, 0)
+    lm, _ = O.local_max_eig(d["jac_state"], s, 0)
+    F_or = O.riemann(s, 0, lm, None, d)
+    F_ref = g[key + f"_flux_{solver}"]
+    got = fl[0]["flux"]
+    scale = np.max(np.abs(F_ref), axis=0) + 1e-300
+    assert np.max(np.abs(got - F_or) / scale) <= 1e-14
+    assert np.max(np.abs(got - F_ref) / scale) <= 1e-13
+    assert abs(fl[0]["eigmax"] - float(g[key + "_eigmax"])) <= 1e-14 * float(g[key + "_eigmax"])
+
+
+@pytest.mark.parametrize("case", [("ppm", "hllc", "wrap"), ("plm", "hllc", "edge"), ("ppm", "llf", "edge")])
+@pytest.mark.parametrize("parity", [0, 1])
+def test_interface_flux_2d(case, parity):
+    """2D double solve + transverse correction (space_2d.npz) at both step
+    parities (the sweep-order flip changes the HLLC normal index)."""
+    from astrea_amd.plan import Plan
+    subgrid, solver, bc = case
+    g = load("space_2d.npz")
+    key = f"{subgrid}_{solver}_{bc}_p{parity}"
+    grid = g[key + "_grid"]
+    N = grid.shape[0]
+    _say(f"[gpu] flux2d {key}")
+    plan = Plan(2, subgrid, solver, "euler", [dict(n0=N, n1=N, boundary=bc, dx=1 / 12, gamma=1.4, cfl=.4, t_end=1.)],
+                varmask=0x17)
+    plan.load([grid])
+    fl = plan.evolve_space(parity)
+    plan.close()
+    for ax in (0, 1):
+        ref = g[key + f"_flux_ax{ax}"]
+        scale = np.max(np.abs(ref), axis=(0, 1)) + 1e-300
+        err = np.max(np.abs(fl[ax]["flux"] - ref) / scale)
+        _say(f"[gpu]   axis {ax}: max scaled flux error {err:.2e}")
+        assert err <= 1e-12
+        e_ref = float(g[key + f"_eigmax_ax{ax}"])
+        assert abs(fl[ax]["eigmax"] - e_ref) <= 1e-14 * e_ref
+
+
+def test_failure_flag_keeps_last_good_state():
+    """The seeded Riemann problem on which the reference raises in step
+    steps_ok + 1: the device reports failure after steps_ok good steps and
+    keeps that state."""
+    m = meta()["cases"]["rp1d_fail"]
+    g = load("steps.npz")
+    grid0 = g["rp1d_fail_grid0"]
+    N = grid0.shape[0]
+    mm = dict(dimension=1, subgrid="ppm", solver="hllc", timestep="ssprk(3,3)", boundary="edge", gamma=1.4, cfl=.5,
+              dx=1 / N, t_end=m["t_end"], checkpoints=1, max_steps=None, magnetic_2d=False)
+    _say("[gpu] failure flag rp1d_fail")
+    out, st = gpu_run(grid0, mm)
+    assert st["failed"] and not st["active"]
+    assert st["steps"] == m["steps_ok"]
+    ref = oracle_run(grid0, dict(mm, max_steps=m["steps_ok"]))
+    assert np.max(rel_l1(out, ref.grid)) <= 1e-12
+
+
+@pytest.mark.parametrize("subgrid,solver", [("ppm", "hllc"), ("plm", "llf"), ("weno5", "hllc")])
+def test_conservation_periodic_2d(subgrid, solver):
+    rng = np.random.default_rng(7)
+    N = 32
+    c = (np.arange(N) + .5) / N
+    x, y = np.meshgrid(c, c, indexing="ij")
+    w = np.zeros((N, N, 8))
+    w[..., 0] = 1 + .3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    w[..., 1] = .2 * np.cos(2 * np.pi * y) + .01 * rng.uniform(-1, 1, (N, N))
+    w[..., 2] = .1 * np.sin(2 * np.pi * x)
+    w[..., 4] = 1.5
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver=solver, boundary="wrap", dx=1 / N, cfl=.4)
+    q0 = O.initial_average(w, s)
+    m = dict(dimension=2, subgrid=subgrid, solver=solver, timestep="ssprk(3,3)", boundary="wrap", gamma=1.4,
+             cfl=.4, dx=1 / N, t_end=10., checkpoints=1, max_steps=10, magnetic_2d=False)
+    _say(f"[gpu] conservation {subgrid} {solver}")
+    out, st = gpu_run(q0, m)
+    assert st["steps"] == 10
+    tot0, tot1 = q0.sum((0, 1)), out.sum((0, 1))
+    assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.maximum(1, np.abs(tot0)))
