@@ -85,7 +85,27 @@ struct ah_plan {
     void* wR = nullptr;
     void* bn = nullptr;
     bool bound = false;
+    // kernel timing (ah_plan_profile)
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kind;
+    int ev_used = 0;
 };
+
+namespace {
+struct ProfScope {
+    ah_plan* p;
+    hipStream_t s;
+    bool on;
+    ProfScope(ah_plan* p_, int kind, hipStream_t s_) : p(p_), s(s_), on(false) {
+        if (p->ev.empty() || 2 * (p->ev_used + 1) > (int)p->ev.size()) return;
+        on = hipEventRecord(p->ev[2 * p->ev_used], s) == hipSuccess;
+        if (on) p->ev_kind[p->ev_used] = kind;
+    }
+    ~ProfScope() {
+        if (on && hipEventRecord(p->ev[2 * p->ev_used + 1], s) == hipSuccess) p->ev_used++;
+    }
+};
+}  // namespace
 
 extern "C" {
 
@@ -209,6 +229,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
 
 int ah_plan_destroy(ah_plan* p) {
     if (!p) return AH_ERR_INVALID;
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     void* bufs[] = {p->d_pd, p->d_ps, p->d_cell_prob, p->d_t1, p->d_tr[0], p->d_tr[1], p->d_tf[0], p->d_tf[1]};
     int rc = AH_OK;
     for (void* b : bufs)
@@ -318,6 +339,7 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.tiles = t1;
         a.flux_out = flux0;
         a.flux_plane = p->probs[0].n0 + 1;
+        ProfScope ps_(p, AH_K_SWEEP1D, s);
         p->ks.sweep1d(a, n1, s);
         return;
     }
@@ -326,7 +348,10 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
     a.bn = p->sch.solver == AH_HLLD ? p->bn : nullptr;
     for (int ax = 0; ax < 2; ++ax) {
         a.tiles = tr[ax];
-        p->ks.recon[ax](a, nr[ax], s);
+        {
+            ProfScope ps_(p, ax == 0 ? AH_K_RECON0 : AH_K_RECON1, s);
+            p->ks.recon[ax](a, nr[ax], s);
+        }
         a.tiles = tf[ax];
         a.last = ax == 1;
         a.out = ax == 0 ? p->Dbuf : Lout;
@@ -334,6 +359,7 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.flux_out = ax == 0 ? flux0 : flux1;
         a.flux_plane = ax == 0 ? (long long)(p->probs[0].n0 + 1) * p->probs[0].n1
                                : (long long)(p->probs[0].n1 + 1) * p->probs[0].n0;
+        ProfScope ps_(p, ax == 0 ? AH_K_FLUX0 : AH_K_FLUX1, s);
         p->ks.flux[ax](a, nf[ax], s);
     }
 }
@@ -349,6 +375,7 @@ static void launch_combine(ah_plan* p, int op, hipStream_t s) {
     c.nv = p->nv;
     c.op = op;
     const long long blocks = std::min<long long>((p->total + 255) / 256, 8192);
+    ProfScope ps_(p, AH_K_COMBINE, s);
     hipLaunchKernelGGL(combine_kernel, dim3((unsigned)blocks), dim3(256), 0, s, c);
 }
 
@@ -364,13 +391,47 @@ int ah_plan_step(ah_plan* p, int nsteps, void* stream) {
             const Stage& S = p->prog[st];
             launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, p->d_t1, p->n1t, tr, p->nrt, tf,
                          p->nft, s);
-            if (st == 0)
+            if (st == 0) {
+                ProfScope ps_(p, AH_K_DT, s);
                 hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
+            }
             launch_combine(p, S.op, s);
         }
+        ProfScope ps_(p, AH_K_ADVANCE, s);
         hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
     }
     HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+int ah_plan_profile(ah_plan* p, int capacity) {
+    if (!p || capacity < 0) return AH_ERR_INVALID;
+    for (hipEvent_t e : p->ev)
+        if (hipEventDestroy(e) != hipSuccess) return AH_ERR_HIP;
+    p->ev.clear();
+    p->ev_kind.clear();
+    p->ev_used = 0;
+    if (capacity == 0) return AH_OK;
+    p->ev.resize(2 * (size_t)capacity);
+    p->ev_kind.resize(capacity);
+    for (auto& e : p->ev) HIPCHK(hipEventCreate(&e));
+    return AH_OK;
+}
+
+int ah_plan_profile_read(ah_plan* p, double* ms, int* counts) {
+    if (!p || !ms || !counts) return AH_ERR_INVALID;
+    for (int k = 0; k < AH_NKIND; ++k) {
+        ms[k] = 0.0;
+        counts[k] = 0;
+    }
+    if (p->ev_used > 0) HIPCHK(hipEventSynchronize(p->ev[2 * p->ev_used - 1]));
+    for (int i = 0; i < p->ev_used; ++i) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, p->ev[2 * i], p->ev[2 * i + 1]));
+        ms[p->ev_kind[i]] += t;
+        counts[p->ev_kind[i]] += 1;
+    }
+    p->ev_used = 0;
     return AH_OK;
 }
 

@@ -100,7 +100,19 @@ class Plan:
 
     # ---- state in/out --------------------------------------------------------
     def load(self, grids):
-        """Pack conservative states given in the reference layout."""
+        """Pack conservative states given in the reference layout (numpy
+        arrays, or CUDA tensors packed on the device)."""
+        if all(isinstance(g, torch.Tensor) for g in grids):
+            for i, g in enumerate(grids):
+                assert tuple(g.shape) == self.shape(i) + (8,), (g.shape, self.shape(i))
+                flat = g.reshape(-1, 8).to(self.device, torch.float64)
+                o, n = self.offsets[i], flat.shape[0]
+                dead = [c for c in range(8) if c not in self.varidx]
+                if dead and bool((flat[:, dead] != 0).any()):
+                    raise ValueError(f"problem {i} has nonzero components {dead} outside the plan's variable set")
+                for v, c in enumerate(self.varidx):
+                    self.state[v, o:o + n] = flat[:, c]
+            return
         host = np.zeros((self.nv, self.total))
         for i, g in enumerate(grids):
             g = np.asarray(g, dtype=np.float64)
@@ -148,6 +160,16 @@ class Plan:
             if not any(s["active"] for s in st):
                 return st
         return self.status()
+
+    # ---- kernel timing ---------------------------------------------------------
+    def profile(self, capacity):
+        L.check(self.lib.ah_plan_profile(self.h, int(capacity)), "ah_plan_profile")
+
+    def profile_read(self):
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_int * 8)()
+        L.check(self.lib.ah_plan_profile_read(self.h, ms, cnt), "ah_plan_profile_read")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(L.KERNEL_KINDS) if cnt[i]}
 
     # ---- parity entry points (single-problem plans) ----------------------------
     def evolve_space(self, parity=0, state=None):

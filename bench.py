@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the astrea finite-volume update on one MI355X.
+
+Workload (default): the 2D shear layer at 1024 x 1024, periodic, gamma 1.4,
+PPM (McCorquodale-Colella limiter) with the slope flattener, HLLC, SSPRK(3,3),
+CFL 0.4 -- one "step" is one full time step of that grid (3 RK stages x 2
+sweeps), inputs resident in HBM.  Two device paths run from the same initial
+state: the plain FP64 path and the split path (FP32 reconstruction, limiter
+and Riemann fluxes; FP64 state, stage combinations, pressure recovery and dt
+reduction).  ``value`` is the zone-updates/s of the faster path whose relative
+L1 error against the FP64 path after the same steps is <= 1e-5 (the FP64
+device path equals the numpy oracle to round-off, see tests/).
+
+Prints one JSON line.  Only one GPU is used: --gpus N with N > 1 exits.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s at L1≤1e-5 vs FP64 oracle; p50/p99 solve latency vs plain"
+HBM_PEAK_GBPS = 8000.0
+SEED = 20261202
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1024, help="grid cells per axis")
+    ap.add_argument("--paths", default="fp64,split")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def cpu_baseline(n, steps):
+    """The oracle (numpy restatement of the reference, with the reference's
+    LAPACK eigenvalue solve per interface) on a bounded sample of the same
+    workload, single-threaded BLAS, in a child process."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from oracle import astrea_oracle as O
+from astrea_amd import ic
+import torch
+w = ic.shear_layer({n}, {SEED})
+q = ic.initial_average(torch.from_numpy(w), 1.4, 'wrap', True).numpy()
+s = O.Scheme(dimension=2, subgrid='ppm', solver='hllc', timestep='ssprk(3,3)', boundary='wrap', gamma=1.4,
+             cfl=.4, dx=1/{n}, flattener='astrea', eig='lapack')
+t0 = time.perf_counter()
+r = O.run(q, s, 1e9, max_steps={steps})
+dt = time.perf_counter() - t0
+print(json.dumps(dict(seconds=dt, zu=r.steps * {n} * {n})))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               PYTHONWARNINGS="ignore")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+    if out.returncode != 0:
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=r["zu"] / r["seconds"], unit="zone-updates/s", cores=1, kind="port",
+                sample=f"oracle (numpy restatement, LAPACK eigvals per interface) on the same shear-layer "
+                       f"problem at {n}x{n}, {steps} steps, {r['seconds']:.1f} s, 1 BLAS thread, CPU: {cpu}")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a design "
+              "note in DESIGN.md, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from astrea_amd import ic
+    from astrea_amd.plan import Plan
+
+    n = args.n
+    dev = torch.device("cuda:0")
+    q0 = ic.to_device(ic.shear_layer(n, SEED), 1.4, "wrap", True)
+    prob = dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9,
+                max_steps=args.warmup + args.steps)
+    results = {}
+    launches_per_step = 3 * 4 + 3 + 2
+    for path in args.paths.split(","):
+        plan = Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], flattener="astrea", precision=path, varmask=0x17)
+        plan.load([q0])
+        plan.reset()
+        plan.step(args.warmup)
+        torch.cuda.synchronize()
+        plan.profile(launches_per_step * args.steps + 8)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record()
+        plan.step(args.steps)
+        stop.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = start.elapsed_time(stop)
+        prof = plan.profile_read()
+        st = plan.status()[0]
+        assert st["steps"] == args.warmup + args.steps and not st["failed"], st
+        results[path] = dict(ms=ms, wall=wall, prof=prof, state=plan.state.clone(), plan=plan)
+    ref = results.get("fp64")
+    l1 = {}
+    for path, r in results.items():
+        if ref is not None and path != "fp64":
+            a, b = r["state"], ref["state"]
+            num = (a - b).abs().sum(dim=1)
+            den = b.abs().sum(dim=1)
+            l1[path] = float(torch.max(num / den).item())
+    ok_paths = [p for p in results if p == "fp64" or l1.get(p, 1.0) <= 1e-5]
+    best = min(ok_paths, key=lambda p: results[p]["ms"])
+    r = results[best]
+    zu = n * n * args.steps
+    ms_per_step = r["ms"] / args.steps
+    value = zu / (r["ms"] / 1e3)
+    # dominant kernel and its roofline (algorithmic bytes: 8 state touches x 4
+    # live components x 8 B per zone-update, the step's 6 sweep-stages sharing
+    # them equally; the flux kernel of a sweep-stage carries its share)
+    kinds = {k: v for k, v in r["prof"].items()}
+    dom = max(kinds, key=lambda k: kinds[k][0])
+    dom_ms = kinds[dom][0] / kinds[dom][1]
+    b_alg_zu = 8 * 4 * 8
+    per_launch_bytes = b_alg_zu * n * n / 6
+    achieved = per_launch_bytes / (dom_ms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_shear1024.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D shear layer {n}x{n} periodic, PPM+flattener(astrea)/HLLC/SSPRK(3,3), CFL 0.4",
+                   "path": best, "cells": n * n, "seed": SEED},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": dom,
+                     "kernel_avg_ms": dom_ms, "bytes_per_launch": per_launch_bytes},
+        "cpu_baseline": cpu,
+        "paths": {p: {"zu_per_s": zu / (v["ms"] / 1e3), "ms_per_step": v["ms"] / args.steps,
+                      "l1_vs_fp64": l1.get(p, 0.0),
+                      "kernels_avg_ms": {k: kv[0] / kv[1] for k, kv in v["prof"].items()}}
+                  for p, v in results.items()},
+        "step_hbm_achieved_GBps": b_alg_zu * value / 1e9,
+    }
+    print(json.dumps(line))
+    for v in results.values():
+        v["plan"].close()
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,19 @@ int ah_evolve_space(ah_plan* plan, int prob, int parity, const double* grid,
 int ah_compute_L(ah_plan* plan, int prob, int parity, const double* flux0, const double* flux1,
                  double* L, void* stream);
 
+/* Kernel timing with HIP events inside ah_plan_step (no reference
+ * counterpart: the reference's only timing is perf_counter around a whole run,
+ * astrea.py:178-180).  ah_plan_profile(plan, capacity) enables recording of up
+ * to `capacity` launches (0 disables); ah_plan_profile_read synchronises,
+ * returns the summed milliseconds and launch counts per kernel kind and
+ * clears the record. */
+enum ah_kernel_kind {
+    AH_K_SWEEP1D = 0, AH_K_RECON0 = 1, AH_K_FLUX0 = 2, AH_K_RECON1 = 3, AH_K_FLUX1 = 4,
+    AH_K_COMBINE = 5, AH_K_DT = 6, AH_K_ADVANCE = 7, AH_NKIND = 8
+};
+int ah_plan_profile(ah_plan* plan, int capacity);
+int ah_plan_profile_read(ah_plan* plan, double* ms, int* counts);
+
 #ifdef __cplusplus
 }
 #endif
