@@ -18,6 +18,7 @@ AH_FLAT_OFF, AH_FLAT_ASTREA, AH_FLAT_COLELLA90 = range(3)
 AH_LLF, AH_HLLC, AH_HLLD = range(3)
 AH_EULER, AH_RK4, AH_SSPRK22, AH_SSPRK33, AH_SSPRK43, AH_SSPRK53, AH_SSPRK54, AH_SSPRK104 = range(8)
 AH_FP64, AH_SPLIT = 0, 1
+AH_COMPAT_ASTREA, AH_COMPAT_PHYSICAL = 0, 1
 AH_VARS_H1, AH_VARS_H2, AH_VARS_M6, AH_VARS_ALL = 0x13, 0x17, 0x77, 0xFF
 
 # every extern "C" symbol of include/astrea_hip.h
@@ -31,7 +32,8 @@ class AhScheme(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int), ("recon", ctypes.c_int), ("limiter", ctypes.c_int),
                 ("flattener", ctypes.c_int), ("solver", ctypes.c_int), ("rk", ctypes.c_int),
                 ("precision", ctypes.c_int), ("varmask", ctypes.c_int), ("magnetic_2d", ctypes.c_int),
-                ("tile_s", ctypes.c_int), ("tile_t", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("tile_s", ctypes.c_int), ("tile_t", ctypes.c_int), ("compat", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class AhProblem(ctypes.Structure):

@@ -323,6 +323,7 @@ static SweepArgs base_args(ah_plan* p) {
     a.limiter = p->sch.limiter;
     a.flattener = p->sch.flattener;
     a.parity_override = -1;
+    a.compat = p->sch.compat;
     return a;
 }
 

@@ -393,10 +393,11 @@ __device__ inline V8<R> flux_llf(const Iface<R>& s, R lam) {
 }
 
 // HLLC (solvers.py:92-138).  sax = the solver's axis argument (the
-// enumeration index of the sweep, solvers.py:34-36); the star momentum is
-// written at index 1 whatever the axis (127-128); F(w+) is the default (133).
+// enumeration index of the sweep, solvers.py:34-36); the reference writes the
+// star momentum at index 1 whatever the axis (127-128), `mom` selects it;
+// F(w+) is the default (133).
 template <class R, int MASK>
-__device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma) {
+__device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma, int mom) {
     using V = VS<MASK>;
     const R rhoL = s.wm.c[0], pL = s.wm.c[4], rhoR = s.wp.c[0], pR = s.wp.c[4];
     const R uL = sax == 0 ? s.wm.c[1] : (V::live(2) ? s.wm.c[2] : R(0));
@@ -426,7 +427,7 @@ __device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma) {
     for (int i = 0; i < 8; ++i) {
         if (!V::live(i)) continue;
         R qs = Q.c[i] * k;
-        if (i == 1) qs = rho * k * sM;
+        if (i == mom) qs = rho * k * sM;
         if (i == 4) qs = qs + k * (sM - u) * (rho * sM + gdiv(p, sS - u));
         F.c[i] = Fb.c[i] + (qs - Q.c[i]) * sS;
     }

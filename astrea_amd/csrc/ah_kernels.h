@@ -52,6 +52,7 @@ struct SweepArgs {
     int flattener;
     int parity_override;  // -1: use per-problem parity
     int last;             // 2D: this flux launch finishes L
+    int compat;           // 0: reference solver axis (enumeration index), 1: physical axis
 };
 
 __device__ inline int clampi(int p, int lo, int hi) { return p < lo ? lo : (p > hi ? hi : p); }
@@ -226,11 +227,11 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
 }
 
 template <class R, int MASK, int SOLVER, int AX>
-__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma) {
+__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma, int mom = 1) {
     if constexpr (SOLVER == LLF)
         return flux_llf<R, MASK>(s, lam);
     else if constexpr (SOLVER == HLLC)
-        return flux_hllc<R, MASK>(s, sax, gamma);
+        return flux_hllc<R, MASK>(s, sax, gamma, mom);
     else
         return sax == 0 ? flux_hlld<R, MASK, 0>(s, Bn, gamma) : flux_hlld<R, MASK, 1>(s, Bn, gamma);
 }
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
             // (solvers.py:160,168,183)
             if (a.bn) {
                 const int parity = a.parity_override >= 0 ? a.parity_override : ps->parity;
-                const int sax = parity == 0 ? AX : 1 - AX;
+                const int sax = a.compat ? AX : (parity == 0 ? AX : 1 - AX);
                 const V8<R> z = W(s);
                 reinterpret_cast<R*>(a.bn)[o] = sax == 0 ? z.c[5] : z.c[6];
             }
@@ -630,7 +631,9 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
     const R gR = R(pd.gamma);
     const long long off = pd.off, P = a.plane;
     const int parity = a.parity_override >= 0 ? a.parity_override : ps->parity;
-    const int sax = parity == 0 ? AX : 1 - AX;  // enumeration index (solvers.py:34-36)
+    // solver axis: the enumeration index of the sweep (solvers.py:34-36), or
+    // the physical axis in the physical-compat mode
+    const int sax = a.compat ? AX : (parity == 0 ? AX : 1 - AX);
     auto et = [&](int p) { return bc == EDGE ? clampi(p, 0, nt - 1) : p; };
     const int bt = t0 - 2, bta = t0 - 1;
     const R* wLg = reinterpret_cast<const R*>(a.wL);
@@ -747,7 +750,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             bn_s[ia][ik] = Bn;
         }
         const R lam = nmax(rad_s[ia][ik], rad_s[ia][ik + 1]);
-        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR);
+        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, a.compat ? sax + 1 : 1);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) {
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
         const R lam = nmax(rad_s[ia0][ik], rad_s[ia0][ik + 1]);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = bn_s[ia0][ik];
-        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR);
+        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, a.compat ? sax + 1 : 1);
         const int k = s0 + ik;
 #pragma unroll
         for (int c = 0; c < 8; ++c)

@@ -55,7 +55,14 @@ enum ah_rk {
     AH_EULER = 0, AH_RK4 = 1, AH_SSPRK22 = 2, AH_SSPRK33 = 3, AH_SSPRK43 = 4,
     AH_SSPRK53 = 5, AH_SSPRK54 = 6, AH_SSPRK104 = 7
 };
-enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };  /* SPLIT: FP32 reconstruction/limiter/Riemann, FP64 state */
+enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };
+/* AH_COMPAT_ASTREA reproduces the reference's Riemann-solver axis argument:
+ * the enumeration index of the sweep, which swaps the normal velocity of
+ * HLLC/HLLD between the axes on every other step (solvers.py:34-36,63,
+ * astrea.py:85), with the HLLC star momentum always at index 1
+ * (solvers.py:127-128).  AH_COMPAT_PHYSICAL passes the physical sweep axis
+ * and writes the star momentum at that axis. */
+enum ah_compat { AH_COMPAT_ASTREA = 0, AH_COMPAT_PHYSICAL = 1 };  /* SPLIT: FP32 reconstruction/limiter/Riemann, FP64 state */
 enum ah_varmask {
     AH_VARS_H1 = 0x13,  /* rho, rho*vx, E          : 1D hydrodynamics */
     AH_VARS_H2 = 0x17,  /* rho, rho*vx, rho*vy, E  : 2D hydrodynamics */
@@ -75,7 +82,8 @@ typedef struct ah_scheme {
     int magnetic_2d; /* constrained transport (evolvers.py:26-32, 51-58, 63-67, 73-76) */
     int tile_s;      /* sweep tile length in cells (0: default / autotuned) */
     int tile_t;      /* transverse tile width in lines, 2D only (0: default) */
-    int reserved[5];
+    int compat;      /* ah_compat */
+    int reserved[4];
 } ah_scheme;
 
 typedef struct ah_problem {

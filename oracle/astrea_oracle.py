@@ -69,6 +69,7 @@ class Scheme:
     flattener: str = "off"        # off | astrea | colella90 (PPM only)
     eig: str = "closed"           # closed | lapack
     parity: int = 0               # step parity: sweep order flips every step
+    compat: str = "astrea"        # astrea: reference quirks | physical: solver axis = sweep axis
 
     @property
     def high_order(self):
@@ -545,7 +546,7 @@ def flux_lw(chars, d):
     return .5 * (fm + fp) - .5 * ((qp - qm) * mx[..., None])
 
 
-def flux_hllc(ax, gamma, d):
+def flux_hllc(ax, gamma, d, mom=1):
     """solvers.py:92-138 with ``ax`` the solver's axis argument (the
     enumeration index of the sweep, solvers.py:34-36).  Keeps the reference's
     star momentum at index 1 (127-128) and the F(w+) default (133)."""
@@ -563,8 +564,8 @@ def flux_hllc(ax, gamma, d):
     sM = gdiv(pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR), rhoL * (sL - uL) - rhoR * (sR - uR))
     kL, kR = gdiv(sL - uL, sL - sM), gdiv(sR - uR, sR - sM)
     QLs, QRs = QL * kL[..., None], QR * kR[..., None]
-    QLs[..., 1] = rhoL * kL * sM
-    QRs[..., 1] = rhoR * kR * sM
+    QLs[..., mom] = rhoL * kL * sM
+    QRs[..., mom] = rhoR * kR * sM
     QLs[..., 4] = QLs[..., 4] + kL * (sM - uL) * (rhoL * sM + gdiv(pL, sL - uL))
     QRs[..., 4] = QRs[..., 4] + kR * (sM - uR) * (rhoR * sM + gdiv(pR, sR - uR))
     F = np.copy(fp)
@@ -692,7 +693,9 @@ def flux_es(ax, gamma, d):
 def riemann(s, ax, lmax, chars, d):
     """solvers.py:13-31 dispatch."""
     if s.solver_category == "hll":
-        return flux_hlld(ax, s.gamma, s.boundary, d) if s.solver.endswith("d") else flux_hllc(ax, s.gamma, d)
+        if s.solver.endswith("d"):
+            return flux_hlld(ax, s.gamma, s.boundary, d)
+        return flux_hllc(ax, s.gamma, d, 1 if s.compat == "astrea" else ax + 1)
     if s.solver_category == "complete":
         return flux_es(ax, s.gamma, d)
     if s.solver.endswith("w"):
@@ -812,6 +815,8 @@ def evolve_space(grid, s):
         data[ax] = d
     fluxes = {}
     for enum_ax, ax in enumerate(order):
+        if s.compat == "physical":
+            enum_ax = ax      # the solver sees the physical sweep axis
         d = data[ax]
         lmax, chars = local_max_eig(d["jac_state"], s, ax)
         eigmax = np.max(np.maximum(lmax[:-1], lmax[1:]))

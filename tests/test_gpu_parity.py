@@ -157,3 +157,58 @@ def test_conservation_periodic_2d(subgrid, solver):
     assert st["steps"] == 10
     tot0, tot1 = q0.sum((0, 1)), out.sum((0, 1))
     assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.maximum(1, np.abs(tot0)))
+
+
+def _shear(n, seed=20261202):
+    from astrea_amd import ic
+    import torch
+    w = ic.shear_layer(n, seed)
+    return ic.initial_average(torch.from_numpy(w), 1.4, "wrap", True).numpy()
+
+
+@pytest.mark.parametrize("subgrid,flattener", [("ppm", "astrea"), ("ppm", "colella90"), ("plm", "off")])
+def test_shear_layer_physical_compat(subgrid, flattener):
+    """The benchmark problem at 32x32 with the physical solver axis and the
+    PPM flattener variants: device FP64 equals the oracle."""
+    from astrea_amd.plan import Plan
+    n, steps = 32, 12
+    q0 = _shear(n)
+    _say(f"[gpu] shear layer {n}^2 {subgrid} flattener={flattener} compat=physical")
+    plan = Plan(2, subgrid, "hllc", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=1.4, cfl=.4,
+                                                        t_end=1e9, max_steps=steps)],
+                flattener=flattener, varmask=0x17, compat="physical")
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=steps)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver="hllc", boundary="wrap", gamma=1.4, cfl=.4, dx=1 / n,
+                 flattener=flattener, compat="physical")
+    ref = O.run(q0, s, 1e9, max_steps=steps)
+    assert st["steps"] == steps and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
+def test_split_precision_error():
+    """Split path (FP32 reconstruction/limiter/Riemann, FP64 state): relative
+    L1 against the FP64 path after 50 steps of the shear layer <= 1e-5."""
+    from astrea_amd.plan import Plan
+    n, steps = 128, 50
+    q0 = _shear(n)
+    outs = {}
+    for prec in ("fp64", "split"):
+        _say(f"[gpu] shear layer {n}^2 precision={prec}")
+        plan = Plan(2, "ppm", "hllc", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=1.4, cfl=.4,
+                                                            t_end=1e9, max_steps=steps)],
+                    flattener="astrea", varmask=0x17, compat="physical", precision=prec)
+        plan.load([q0])
+        plan.reset()
+        st = plan.run(chunk=steps)[0]
+        assert st["steps"] == steps and not st["failed"]
+        outs[prec] = plan.unpack(0)
+        plan.close()
+    e = np.max(rel_l1(outs["split"], outs["fp64"]))
+    _say(f"[gpu]   split vs fp64 rel L1 {e:.2e}")
+    assert e <= 1e-5
