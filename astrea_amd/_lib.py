@@ -24,7 +24,8 @@ AH_VARS_H1, AH_VARS_H2, AH_VARS_M6, AH_VARS_ALL = 0x13, 0x17, 0x77, 0xFF
 # every extern "C" symbol of include/astrea_hip.h
 EXPORTS = ["ah_version", "ah_status_str", "ah_plan_create", "ah_plan_destroy", "ah_plan_layout",
            "ah_plan_workspace_size", "ah_plan_bind", "ah_plan_reset", "ah_plan_step", "ah_plan_status",
-           "ah_evolve_space", "ah_compute_L", "ah_plan_profile", "ah_plan_profile_read"]
+           "ah_evolve_space", "ah_compute_L", "ah_plan_profile", "ah_plan_profile_read", "ah_plan_step_graph",
+           "ah_plan_compact", "ah_plan_set_blocks"]
 KERNEL_KINDS = ["sweep1d", "recon_ax0", "flux_ax0", "recon_ax1", "flux_ax1", "combine", "dt", "advance"]
 
 
@@ -79,6 +80,9 @@ def load():
     lib.ah_evolve_space.argtypes = [vp, i, i, vp, vp, vp, P(d), vp]
     lib.ah_compute_L.argtypes = [vp, i, i, vp, vp, vp, vp]
     lib.ah_plan_profile.argtypes = [vp, i]
+    lib.ah_plan_step_graph.argtypes = [vp, i, i, vp]
+    lib.ah_plan_compact.argtypes = [vp, P(i), vp]
+    lib.ah_plan_set_blocks.argtypes = [vp, i]
     lib.ah_plan_profile_read.argtypes = [vp, P(d), P(i)]
     for name in EXPORTS:
         if name != "ah_status_str":

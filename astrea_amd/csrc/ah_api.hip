@@ -15,6 +15,15 @@ using namespace ah;
 
 namespace {
 
+// A tile list the kernels walk with a grid-stride loop; the count lives on
+// the device so a captured graph stays valid when the list is compacted.
+struct TileList {
+    std::vector<Tile> full;
+    Tile* d = nullptr;
+    int* d_count = nullptr;
+    int count = 0;
+};
+
 struct Stage {
     int in;  // register holding the stage input (0 = state)
     int op;  // RkOp applied after the stage's L
@@ -71,11 +80,12 @@ struct ah_plan {
     // device tables
     ProbDesc* d_pd = nullptr;
     ProbState* d_ps = nullptr;
-    int* d_cell_prob = nullptr;
-    Tile* d_t1 = nullptr;
-    Tile* d_tr[2] = {nullptr, nullptr};
-    Tile* d_tf[2] = {nullptr, nullptr};
-    int n1t = 0, nrt[2] = {0, 0}, nft[2] = {0, 0};
+    // tile lists: 0 = 1D sweep, 1/2 = recon ax0/ax1, 3/4 = flux ax0/ax1, 5 = cells (combine)
+    TileList tl[6];
+    int max_blocks = 2048;  // persistent grid cap per launch
+    // captured step graphs (ah_plan_step_graph)
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
     // bound memory
     double* U = nullptr;
     double* regs[9] = {nullptr};
@@ -165,8 +175,6 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     // problem tables and tiles
     std::vector<ProbDesc> pd(nprob);
     long long total = 0;
-    std::vector<Tile> t1, tr[2], tf[2];
-    std::vector<int> cell_prob;
     for (int i = 0; i < nprob; ++i) {
         const ah_problem& q = probs[i];
         const int n1 = sch->dims == 1 ? 1 : q.n1;
@@ -186,39 +194,46 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
         pd[i].chkpt = q.t_end / q.checkpoints;
         pd[i].max_steps = q.max_steps;
         p->offs.push_back(total);
-        total += (long long)q.n0 * n1;
+        const long long cells = (long long)q.n0 * n1;
+        total += cells;
         if (sch->dims == 1) {
-            for (int s0 = 0; s0 < q.n0; s0 += p->ks.ts1) t1.push_back({i, s0, 0, 0});
+            for (int s0 = 0; s0 < q.n0; s0 += p->ks.ts1) p->tl[0].full.push_back({i, s0, 0, 0});
         } else {
             for (int ax = 0; ax < 2; ++ax) {
                 const int ns = ax == 0 ? q.n0 : n1, nt = ax == 0 ? n1 : q.n0;
                 for (int t0 = 0; t0 < nt; t0 += p->ks.rtt[ax])
-                    for (int s0 = 0; s0 < ns; s0 += p->ks.rts[ax]) tr[ax].push_back({i, s0, t0, 0});
+                    for (int s0 = 0; s0 < ns; s0 += p->ks.rts[ax]) p->tl[1 + ax].full.push_back({i, s0, t0, 0});
                 for (int t0 = 0; t0 < nt; t0 += p->ks.ftt[ax])
-                    for (int s0 = 0; s0 < ns; s0 += p->ks.fts[ax]) tf[ax].push_back({i, s0, t0, 0});
+                    for (int s0 = 0; s0 < ns; s0 += p->ks.fts[ax]) p->tl[3 + ax].full.push_back({i, s0, t0, 0});
             }
         }
+        for (long long c0 = 0; c0 < cells; c0 += 1024)
+            p->tl[5].full.push_back({i, (int)c0, (int)std::min<long long>(1024, cells - c0), 0});
     }
     p->total = total;
-    if (nprob > 1) {
-        cell_prob.resize(total);
-        for (int i = 0; i < nprob; ++i) std::fill(cell_prob.begin() + pd[i].off, cell_prob.begin() + pd[i].off +
-                                                                                     (long long)pd[i].n0 * pd[i].n1, i);
+    if (total >= (1ll << 31)) {
+        delete p;
+        return AH_ERR_INVALID;
     }
-    auto upload = [](auto** dst, const auto& v) -> bool {
-        if (v.empty()) return true;
-        if (hipMalloc((void**)dst, v.size() * sizeof(v[0])) != hipSuccess) return false;
-        return hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice) == hipSuccess;
-    };
-    bool up = upload(&p->d_pd, pd) && upload(&p->d_t1, t1) && upload(&p->d_tr[0], tr[0]) && upload(&p->d_tr[1], tr[1]) &&
-              upload(&p->d_tf[0], tf[0]) && upload(&p->d_tf[1], tf[1]) && upload(&p->d_cell_prob, cell_prob);
-    up = up && hipMalloc((void**)&p->d_ps, nprob * sizeof(ProbState)) == hipSuccess &&
-         hipMemset(p->d_ps, 0, nprob * sizeof(ProbState)) == hipSuccess;
-    p->n1t = (int)t1.size();
-    for (int ax = 0; ax < 2; ++ax) {
-        p->nrt[ax] = (int)tr[ax].size();
-        p->nft[ax] = (int)tf[ax].size();
+    bool up = hipMalloc((void**)&p->d_pd, nprob * sizeof(ProbDesc)) == hipSuccess &&
+              hipMemcpy(p->d_pd, pd.data(), nprob * sizeof(ProbDesc), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc((void**)&p->d_ps, nprob * sizeof(ProbState)) == hipSuccess &&
+              hipMemset(p->d_ps, 0, nprob * sizeof(ProbState)) == hipSuccess;
+    for (TileList& L : p->tl) {
+        if (!up) break;
+        L.count = (int)L.full.size();
+        up = hipMalloc((void**)&L.d_count, sizeof(int)) == hipSuccess &&
+             hipMemcpy(L.d_count, &L.count, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+        if (up && L.count)
+            up = hipMalloc((void**)&L.d, L.full.size() * sizeof(Tile)) == hipSuccess &&
+                 hipMemcpy(L.d, L.full.data(), L.full.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess;
     }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    p->max_blocks = 8 * cus;
     if (!up) {
         ah_plan_destroy(p);
         return AH_ERR_HIP;
@@ -230,10 +245,15 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
 int ah_plan_destroy(ah_plan* p) {
     if (!p) return AH_ERR_INVALID;
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
-    void* bufs[] = {p->d_pd, p->d_ps, p->d_cell_prob, p->d_t1, p->d_tr[0], p->d_tr[1], p->d_tf[0], p->d_tf[1]};
     int rc = AH_OK;
+    if (p->graph_exec && hipGraphExecDestroy(p->graph_exec) != hipSuccess) rc = AH_ERR_HIP;
+    void* bufs[] = {p->d_pd, p->d_ps};
     for (void* b : bufs)
         if (b && hipFree(b) != hipSuccess) rc = AH_ERR_HIP;
+    for (TileList& L : p->tl) {
+        if (L.d && hipFree(L.d) != hipSuccess) rc = AH_ERR_HIP;
+        if (L.d_count && hipFree(L.d_count) != hipSuccess) rc = AH_ERR_HIP;
+    }
     delete p;
     return rc;
 }
@@ -309,6 +329,13 @@ int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
 int ah_plan_reset(ah_plan* p, void* stream) {
     if (!p) return AH_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipStreamSynchronize(s));
+    for (TileList& L : p->tl) {  // every problem is active again: full work lists
+        if (L.full.empty()) continue;
+        L.count = (int)L.full.size();
+        HIPCHK(hipMemcpy(L.d, L.full.data(), L.full.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.d_count, &L.count, sizeof(int), hipMemcpyHostToDevice));
+    }
     hipLaunchKernelGGL(reset_kernel, dim3((p->nprob + 255) / 256), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
     HIPCHK(hipGetLastError());
     return AH_OK;
@@ -327,33 +354,37 @@ static SweepArgs base_args(ah_plan* p) {
     return a;
 }
 
+static int blocks_for(const ah_plan* p, const TileList& L) { return std::min<int>((int)L.full.size(), p->max_blocks); }
+
 // L of the stage input X for every active problem (evolve_space + compute_L)
 static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, int parity_override,
-                         double* flux0, double* flux1, const Tile* t1, int n1, const Tile* const* tr, const int* nr,
-                         const Tile* const* tf, const int* nf, hipStream_t s) {
+                         double* flux0, double* flux1, hipStream_t s) {
     SweepArgs a = base_args(p);
     a.X = X;
     a.stage0 = stage0;
     a.parity_override = parity_override;
     if (p->sch.dims == 1) {
         a.out = Lout;
-        a.tiles = t1;
+        a.tiles = p->tl[0].d;
+        a.ntiles_dev = p->tl[0].d_count;
         a.flux_out = flux0;
         a.flux_plane = p->probs[0].n0 + 1;
         ProfScope ps_(p, AH_K_SWEEP1D, s);
-        p->ks.sweep1d(a, n1, s);
+        p->ks.sweep1d(a, blocks_for(p, p->tl[0]), s);
         return;
     }
     a.wL = p->wL;
     a.wR = p->wR;
     a.bn = p->sch.solver == AH_HLLD ? p->bn : nullptr;
     for (int ax = 0; ax < 2; ++ax) {
-        a.tiles = tr[ax];
+        a.tiles = p->tl[1 + ax].d;
+        a.ntiles_dev = p->tl[1 + ax].d_count;
         {
             ProfScope ps_(p, ax == 0 ? AH_K_RECON0 : AH_K_RECON1, s);
-            p->ks.recon[ax](a, nr[ax], s);
+            p->ks.recon[ax](a, blocks_for(p, p->tl[1 + ax]), s);
         }
-        a.tiles = tf[ax];
+        a.tiles = p->tl[3 + ax].d;
+        a.ntiles_dev = p->tl[3 + ax].d_count;
         a.last = ax == 1;
         a.out = ax == 0 ? p->Dbuf : Lout;
         a.Dprev = p->Dbuf;
@@ -361,7 +392,7 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.flux_plane = ax == 0 ? (long long)(p->probs[0].n0 + 1) * p->probs[0].n1
                                : (long long)(p->probs[0].n1 + 1) * p->probs[0].n0;
         ProfScope ps_(p, ax == 0 ? AH_K_FLUX0 : AH_K_FLUX1, s);
-        p->ks.flux[ax](a, nf[ax], s);
+        p->ks.flux[ax](a, blocks_for(p, p->tl[3 + ax]), s);
     }
 }
 
@@ -371,37 +402,98 @@ static void launch_combine(ah_plan* p, int op, hipStream_t s) {
     c.L = p->Lbuf;
     c.pd = p->d_pd;
     c.ps = p->d_ps;
-    c.cell_prob = p->nprob > 1 ? p->d_cell_prob : nullptr;
+    c.tiles = p->tl[5].d;
+    c.ntiles_dev = p->tl[5].d_count;
     c.plane = p->total;
     c.nv = p->nv;
     c.op = op;
-    const long long blocks = std::min<long long>((p->total + 255) / 256, 8192);
     ProfScope ps_(p, AH_K_COMBINE, s);
-    hipLaunchKernelGGL(combine_kernel, dim3((unsigned)blocks), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(p, p->tl[5])), dim3(256), 0, s, c);
+}
+
+// one full time step of every active problem
+static void launch_step(ah_plan* p, hipStream_t s) {
+    const int pb = (p->nprob + 255) / 256;
+    for (size_t st = 0; st < p->prog.size(); ++st) {
+        const Stage& S = p->prog[st];
+        launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s);
+        if (st == 0) {
+            ProfScope ps_(p, AH_K_DT, s);
+            hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
+        }
+        launch_combine(p, S.op, s);
+    }
+    ProfScope ps_(p, AH_K_ADVANCE, s);
+    hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
 }
 
 int ah_plan_step(ah_plan* p, int nsteps, void* stream) {
     if (!p || nsteps < 0) return AH_ERR_INVALID;
     if (!p->bound) return AH_ERR_STATE;
     hipStream_t s = (hipStream_t)stream;
-    const Tile* tr[2] = {p->d_tr[0], p->d_tr[1]};
-    const Tile* tf[2] = {p->d_tf[0], p->d_tf[1]};
-    const int pb = (p->nprob + 255) / 256;
-    for (int n = 0; n < nsteps; ++n) {
-        for (size_t st = 0; st < p->prog.size(); ++st) {
-            const Stage& S = p->prog[st];
-            launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, p->d_t1, p->n1t, tr, p->nrt, tf,
-                         p->nft, s);
-            if (st == 0) {
-                ProfScope ps_(p, AH_K_DT, s);
-                hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
-            }
-            launch_combine(p, S.op, s);
-        }
-        ProfScope ps_(p, AH_K_ADVANCE, s);
-        hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
-    }
+    for (int n = 0; n < nsteps; ++n) launch_step(p, s);
     HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+int ah_plan_step_graph(ah_plan* p, int nsteps, int nlaunch, void* stream) {
+    if (!p || nsteps <= 0 || nlaunch < 0 || !stream) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    if (!p->graph_exec || p->graph_steps != nsteps) {
+        if (p->graph_exec) {
+            HIPCHK(hipGraphExecDestroy(p->graph_exec));
+            p->graph_exec = nullptr;
+        }
+        std::vector<hipEvent_t> saved;
+        saved.swap(p->ev);  // no event records inside a captured graph
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int n = 0; n < nsteps; ++n) launch_step(p, s);
+        const hipError_t e1 = hipStreamEndCapture(s, &g);
+        saved.swap(p->ev);
+        if (e1 != hipSuccess || !g) return AH_ERR_HIP;
+        const hipError_t e2 = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            p->graph_exec = nullptr;
+            return AH_ERR_HIP;
+        }
+        p->graph_steps = nsteps;
+    }
+    for (int i = 0; i < nlaunch; ++i) HIPCHK(hipGraphLaunch(p->graph_exec, s));
+    return AH_OK;
+}
+
+int ah_plan_compact(ah_plan* p, int* nactive, void* stream) {
+    if (!p) return AH_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<ProbState> h(p->nprob);
+    HIPCHK(hipMemcpyAsync(h.data(), p->d_ps, p->nprob * sizeof(ProbState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int na = 0;
+    for (const ProbState& q : h) na += q.active ? 1 : 0;
+    if (nactive) *nactive = na;
+    for (TileList& L : p->tl) {
+        if (L.full.empty()) continue;
+        std::vector<Tile> act;
+        act.reserve(L.full.size());
+        for (const Tile& t : L.full)
+            if (h[t.prob].active) act.push_back(t);
+        L.count = (int)act.size();
+        if (L.count) HIPCHK(hipMemcpy(L.d, act.data(), act.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.d_count, &L.count, sizeof(int), hipMemcpyHostToDevice));
+    }
+    return AH_OK;
+}
+
+int ah_plan_set_blocks(ah_plan* p, int max_blocks) {
+    if (!p || max_blocks <= 0) return AH_ERR_INVALID;
+    p->max_blocks = max_blocks;
+    if (p->graph_exec) {
+        HIPCHK(hipGraphExecDestroy(p->graph_exec));
+        p->graph_exec = nullptr;
+    }
     return AH_OK;
 }
 
@@ -466,9 +558,7 @@ int ah_evolve_space(ah_plan* p, int prob, int parity, const double* grid, double
     tmp.eig[0] = tmp.eig[1] = 0ull;
     tmp.parity = parity;
     HIPCHK(hipMemcpyAsync(p->d_ps, &tmp, sizeof(ProbState), hipMemcpyHostToDevice, s));
-    const Tile* tr[2] = {p->d_tr[0], p->d_tr[1]};
-    const Tile* tf[2] = {p->d_tf[0], p->d_tf[1]};
-    launch_space(p, grid, p->Lbuf, 1, parity, flux0, flux1, p->d_t1, p->n1t, tr, p->nrt, tf, p->nft, s);
+    launch_space(p, grid, p->Lbuf, 1, parity, flux0, flux1, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&tmp, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -92,7 +92,8 @@ struct CombineArgs {
     const double* L;
     const ProbDesc* pd;
     const ProbState* ps;
-    const int* cell_prob;  // problem id of every cell (batched); nullptr: single problem 0
+    const Tile* tiles;       // cell tiles: (problem, first cell, cell count)
+    const int* ntiles_dev;
     long long plane;
     int nv;
     int op;
@@ -100,13 +101,16 @@ struct CombineArgs {
 
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
     const long long n = a.plane;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int p = a.cell_prob ? a.cell_prob[i] : 0;
-        const ProbState& s = a.ps[p];
+    const int ntiles = *a.ntiles_dev;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = a.tiles[ti];
+        const ProbState& s = a.ps[tl.prob];
         if (!s.active || s.fail_flag) continue;
         const double dt = s.dt;
+        const long long base = a.pd[tl.prob].off + tl.s0;
+        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x)
         for (int v = 0; v < a.nv; ++v) {
-            const long long j = v * n + i;
+            const long long j = v * n + base + c;
             double* U = a.regs[0];
             double* K1 = a.regs[1];
             double* K2 = a.regs[2];

@@ -46,7 +46,8 @@ struct SweepArgs {
     const ProbDesc* pd;
     ProbState* ps;
     const Tile* tiles;
-    int ntiles;
+    int ntiles;             // tile count when ntiles_dev is null
+    const int* ntiles_dev;  // device-resident tile count (compactable lists)
     int stage0;   // accumulate the eigmax for dt (stage 0 of a step)
     int limiter;  // PLM slope limiter
     int flattener;
@@ -279,10 +280,12 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     __shared__ R fl[NV][NF];
     __shared__ double red[4];
 
-    const Tile tile = a.tiles[blockIdx.x];
+    const int ntiles_ = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
+    const Tile tile = a.tiles[ti_];
     const ProbDesc pd = a.pd[tile.prob];
     ProbState* ps = a.ps + tile.prob;
-    if (!ps->active) return;
+    if (!ps->active) continue;  // uniform for the workgroup
     const int N = pd.n0, bc = pd.bc, s0 = tile.s0;
     const double gamma = pd.gamma;
     const R gR = R(gamma);
@@ -462,6 +465,8 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             a.out[v * P + off + cell] = -double(d);
         }
     }
+    __syncthreads();  // the next tile reuses the LDS windows
+    }
 }
 
 // ============================================================================
@@ -483,10 +488,12 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
     __shared__ double cq[HO ? NV : 1][HO ? NQT : 1][HO ? NQS : 1];
     __shared__ R ws[NV][TT][NWS];
 
-    const Tile tile = a.tiles[blockIdx.x];
+    const int ntiles_ = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
+    const Tile tile = a.tiles[ti_];
     const ProbDesc pd = a.pd[tile.prob];
     ProbState* ps = a.ps + tile.prob;
-    if (!ps->active) return;
+    if (!ps->active) continue;  // uniform for the workgroup
     const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
     const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
     const double gamma = pd.gamma;
@@ -593,6 +600,8 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
             }
         }
     }
+    __syncthreads();  // the next tile reuses the LDS windows
+    }
 }
 
 // ============================================================================
@@ -622,10 +631,12 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
     __shared__ R F_s[NV][TT][NKF];
     __shared__ double red[4];
 
-    const Tile tile = a.tiles[blockIdx.x];
+    const int ntiles_ = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
+    const Tile tile = a.tiles[ti_];
     const ProbDesc pd = a.pd[tile.prob];
     ProbState* ps = a.ps + tile.prob;
-    if (!ps->active) return;
+    if (!ps->active) continue;  // uniform for the workgroup
     const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
     const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
     const R gR = R(pd.gamma);
@@ -825,6 +836,8 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             else
                 a.out[v * P + g] = d;
         }
+    }
+    __syncthreads();  // the next tile reuses the LDS windows
     }
 }
 

@@ -147,6 +147,19 @@ class Plan:
     def step(self, nsteps=1):
         L.check(self.lib.ah_plan_step(self.h, int(nsteps), self._stream()), "ah_plan_step")
 
+    def step_graph(self, nsteps, nlaunch=1):
+        """Replay a captured hipGraph of `nsteps` steps `nlaunch` times (the
+        current torch stream must not be the default stream)."""
+        L.check(self.lib.ah_plan_step_graph(self.h, int(nsteps), int(nlaunch), self._stream()), "ah_plan_step_graph")
+
+    def compact(self):
+        n = ctypes.c_int()
+        L.check(self.lib.ah_plan_compact(self.h, ctypes.byref(n), self._stream()), "ah_plan_compact")
+        return n.value
+
+    def set_blocks(self, nblocks):
+        L.check(self.lib.ah_plan_set_blocks(self.h, int(nblocks)), "ah_plan_set_blocks")
+
     def status(self):
         st = (L.AhProblemStatus * self.nprob)()
         L.check(self.lib.ah_plan_status(self.h, st, self._stream()), "ah_plan_status")

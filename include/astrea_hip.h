@@ -137,6 +137,20 @@ int ah_plan_reset(ah_plan* plan, void* stream);
  * max_steps (or fails) is skipped by the remaining launches. */
 int ah_plan_step(ah_plan* plan, int nsteps, void* stream);
 
+/* The same steps captured once into a hipGraph of `nsteps` steps and
+ * launched `nlaunch` times (reference loop astrea.py:45-85, replayed without
+ * host work).  Needs a non-default stream; the graph stays valid across
+ * ah_plan_compact.  No reference counterpart beyond the loop itself. */
+int ah_plan_step_graph(ah_plan* plan, int nsteps, int nlaunch, void* stream);
+
+/* Drop the tiles of problems that are no longer active from every kernel's
+ * work list (synchronises the stream); `nactive` receives the number of
+ * active problems.  Batched runs call it between chunks of steps. */
+int ah_plan_compact(ah_plan* plan, int* nactive, void* stream);
+
+/* Persistent grid cap: workgroups per launch (each loops over its tiles). */
+int ah_plan_set_blocks(ah_plan* plan, int max_blocks);
+
 /* Copy the per-problem time state to the host (synchronises the stream). */
 int ah_plan_status(ah_plan* plan, ah_problem_status* out, void* stream);
 
