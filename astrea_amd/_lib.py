@@ -25,7 +25,7 @@ AH_VARS_H1, AH_VARS_H2, AH_VARS_M6, AH_VARS_ALL = 0x13, 0x17, 0x77, 0xFF
 EXPORTS = ["ah_version", "ah_status_str", "ah_plan_create", "ah_plan_destroy", "ah_plan_layout",
            "ah_plan_workspace_size", "ah_plan_bind", "ah_plan_reset", "ah_plan_step", "ah_plan_status",
            "ah_evolve_space", "ah_compute_L", "ah_plan_profile", "ah_plan_profile_read", "ah_plan_step_graph",
-           "ah_plan_compact", "ah_plan_set_blocks"]
+           "ah_plan_compact", "ah_plan_set_blocks", "ah_plan_time_export", "ah_plan_time_import"]
 KERNEL_KINDS = ["sweep1d", "recon_ax0", "flux_ax0", "recon_ax1", "flux_ax1", "combine", "dt", "advance"]
 
 
@@ -46,6 +46,11 @@ class AhProblem(ctypes.Structure):
 class AhProblemStatus(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("steps", ctypes.c_longlong),
                 ("active", ctypes.c_int), ("failed", ctypes.c_int)]
+
+
+class AhTimeState(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("steps", ctypes.c_longlong),
+                ("idx", ctypes.c_int), ("active", ctypes.c_int), ("failed", ctypes.c_int), ("parity", ctypes.c_int)]
 
 
 class AstreaHipError(RuntimeError):
@@ -83,6 +88,8 @@ def load():
     lib.ah_plan_step_graph.argtypes = [vp, i, i, vp]
     lib.ah_plan_compact.argtypes = [vp, P(i), vp]
     lib.ah_plan_set_blocks.argtypes = [vp, i]
+    lib.ah_plan_time_export.argtypes = [vp, P(AhTimeState), vp]
+    lib.ah_plan_time_import.argtypes = [vp, P(AhTimeState), vp]
     lib.ah_plan_profile_read.argtypes = [vp, P(d), P(i)]
     for name in EXPORTS:
         if name != "ah_status_str":

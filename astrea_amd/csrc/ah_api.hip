@@ -528,6 +528,36 @@ int ah_plan_profile_read(ah_plan* p, double* ms, int* counts) {
     return AH_OK;
 }
 
+int ah_plan_time_export(ah_plan* p, ah_time_state* out, void* stream) {
+    if (!p || !out) return AH_ERR_INVALID;
+    std::vector<ProbState> h(p->nprob);
+    HIPCHK(hipMemcpyAsync(h.data(), p->d_ps, p->nprob * sizeof(ProbState), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    for (int i = 0; i < p->nprob; ++i)
+        out[i] = ah_time_state{h[i].t, h[i].dt, h[i].steps, h[i].idx, h[i].active, h[i].failed, h[i].parity};
+    return AH_OK;
+}
+
+int ah_plan_time_import(ah_plan* p, const ah_time_state* in, void* stream) {
+    if (!p || !in) return AH_ERR_INVALID;
+    std::vector<ProbState> h(p->nprob);
+    for (int i = 0; i < p->nprob; ++i) {
+        ProbState q;
+        memset(&q, 0, sizeof(q));
+        q.t = in[i].t;
+        q.dt = in[i].dt;
+        q.steps = in[i].steps;
+        q.idx = in[i].idx;
+        q.active = in[i].active;
+        q.failed = in[i].failed;
+        q.parity = in[i].parity;
+        h[i] = q;
+    }
+    HIPCHK(hipMemcpyAsync(p->d_ps, h.data(), p->nprob * sizeof(ProbState), hipMemcpyHostToDevice, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return AH_OK;
+}
+
 int ah_plan_status(ah_plan* p, ah_problem_status* out, void* stream) {
     if (!p || !out) return AH_ERR_INVALID;
     std::vector<ProbState> h(p->nprob);

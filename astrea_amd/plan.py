@@ -152,6 +152,21 @@ class Plan:
         current torch stream must not be the default stream)."""
         L.check(self.lib.ah_plan_step_graph(self.h, int(nsteps), int(nlaunch), self._stream()), "ah_plan_step_graph")
 
+    def time_export(self):
+        arr = (L.AhTimeState * self.nprob)()
+        L.check(self.lib.ah_plan_time_export(self.h, arr, self._stream()), "ah_plan_time_export")
+        return [(a.t, a.dt, a.steps, a.idx, a.active, a.failed, a.parity) for a in arr]
+
+    def time_import(self, states):
+        arr = (L.AhTimeState * self.nprob)()
+        for i, s in enumerate(states):
+            arr[i] = L.AhTimeState(*s)
+        L.check(self.lib.ah_plan_time_import(self.h, arr, self._stream()), "ah_plan_time_import")
+
+    def bytes_per_cell(self):
+        """Device bytes per cell of this plan (state + workspace)."""
+        return (self.state.numel() * 8 + self.workspace.numel()) / max(self.total, 1)
+
     def compact(self):
         n = ctypes.c_int()
         L.check(self.lib.ah_plan_compact(self.h, ctypes.byref(n), self._stream()), "ah_plan_compact")

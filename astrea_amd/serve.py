@@ -1,0 +1,229 @@
+"""Continuous-batching solve server.
+
+Requests (2D problems of different sizes, schemes and end times) arrive over
+time.  Admitted requests keep their state resident on the device; requests
+of one scheme share one batched plan, each with its own t and dt on the
+device.  Admission is against an HBM budget: a request waits until the bytes
+of the active set plus its own fit.  Whenever the active set of a scheme
+changes (an admission or a completion) its plan is rebuilt and the running
+requests' state planes and time states move over (device-to-device copies),
+so new work joins the batch between chunks of steps instead of waiting for
+the batch to drain.  Each scheme group runs on its own stream, so groups
+overlap on the GPU.
+
+``run_fifo`` is the plain comparison: one request at a time, FP64, eager
+launches, in arrival order.
+"""
+import collections
+import time
+from dataclasses import dataclass, field
+
+import torch
+
+from . import ic
+from .plan import Plan
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int
+    subgrid: str
+    arrival: float
+    t_end: float = .25
+    states: list = None
+    grid: torch.Tensor = None          # initial conservative state (n, n, 8), device
+    admitted_at: float = None
+    done_at: float = None
+    status: dict = None
+    result: torch.Tensor = None        # final packed state planes (nv, n*n)
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return None if self.done_at is None else self.done_at - self.arrival
+
+
+def make_requests(nreq, seed, rate, sizes=(128, 256, 512, 1024), device="cuda:0"):
+    """Seeded four-quadrant requests with Poisson arrivals at ``rate`` per second."""
+    import numpy as np
+    specs = ic.quadrant_requests(nreq, seed, sizes)
+    rng = np.random.default_rng(seed + 1)
+    t = 0.0
+    out = []
+    for i, sp in enumerate(specs):
+        t += float(rng.exponential(1.0 / rate))
+        w = ic.quadrants(sp["n"], sp["states"])
+        g = ic.to_device(w, 1.4, "edge", sp["subgrid"] == "ppm", device)
+        out.append(Request(i, sp["n"], sp["subgrid"], t, .25, sp["states"], g))
+    return out
+
+
+def _problem(r, cfl):
+    return dict(n0=r.n, n1=r.n, boundary="edge", dx=1 / r.n, gamma=1.4, cfl=cfl, t_end=r.t_end)
+
+
+class _Group:
+    def __init__(self, subgrid, precision, compat, cfl, device):
+        self.subgrid, self.precision, self.compat, self.cfl = subgrid, precision, compat, cfl
+        self.device = torch.device(device)
+        self.stream = torch.cuda.Stream(device=self.device)
+        self.reqs = []
+        self.plan = None
+        self.stale = False
+
+    def _new_plan(self, reqs):
+        return Plan(2, self.subgrid, "hllc", "ssprk(3,3)", [_problem(r, self.cfl) for r in reqs],
+                    precision=self.precision, varmask=0x17, compat=self.compat, device=self.device)
+
+    def rebuild(self):
+        """New plan over self.reqs; running requests keep state and time."""
+        with torch.cuda.stream(self.stream):
+            old, old_reqs = self.plan, getattr(self, "_planned", [])
+            if not self.reqs:
+                if old:
+                    old.close()
+                self.plan, self._planned, self.stale = None, [], False
+                return
+            new = self._new_plan(self.reqs)
+            times = []
+            old_times = old.time_export() if old else []
+            old_index = {r.rid: i for i, r in enumerate(old_reqs)}
+            for j, r in enumerate(self.reqs):
+                o_new = new.offsets[j]
+                cells = r.n * r.n
+                if r.rid in old_index:
+                    i = old_index[r.rid]
+                    o_old = old.offsets[i]
+                    new.state[:, o_new:o_new + cells].copy_(old.state[:, o_old:o_old + cells])
+                    times.append(old_times[i])
+                else:
+                    flat = r.grid.reshape(-1, 8)
+                    for v, c in enumerate(new.varidx):
+                        new.state[v, o_new:o_new + cells] = flat[:, c]
+                    active = 1 if (0.0 <= r.t_end and r.t_end != 0.0) else 0
+                    times.append((0.0, 0.0, 0, 1, active, 0, 0))
+            new.time_import(times)
+            if old:
+                old.close()
+            self.plan, self._planned, self.stale = new, list(self.reqs), False
+
+    def advance(self, chunk, use_graph=True):
+        if self.plan is None:
+            return
+        with torch.cuda.stream(self.stream):
+            if use_graph:
+                self.plan.step_graph(chunk, 1)
+            else:
+                self.plan.step(chunk)
+
+    def harvest(self, now, keep_results):
+        """Finished requests (done or failed) leave the group."""
+        if self.plan is None:
+            return []
+        with torch.cuda.stream(self.stream):
+            st = self.plan.status()
+        done = []
+        for i, (r, s) in enumerate(zip(self._planned, st)):
+            if not s["active"]:
+                r.done_at = now()
+                r.status = s
+                if keep_results:
+                    o = self.plan.offsets[i]
+                    r.result = self.plan.state[:, o:o + r.n * r.n].clone()
+                done.append(r)
+        if done:
+            ids = {r.rid for r in done}
+            self.reqs = [r for r in self.reqs if r.rid not in ids]
+            self.stale = True
+        return done
+
+
+class Server:
+    def __init__(self, hbm_budget=32 << 30, precision="fp64", compat="physical", cfl=.4, chunk=16,
+                 device="cuda:0", keep_results=False, bytes_per_cell=None):
+        self.budget = hbm_budget
+        self.precision, self.compat, self.cfl, self.chunk = precision, compat, cfl, chunk
+        self.device = device
+        self.keep_results = keep_results
+        self.groups = {}
+        # state + stage registers + interface states per cell (2D hydro, SSPRK(3,3))
+        rb = 8 if precision == "fp64" else 4
+        self.bytes_per_cell = bytes_per_cell or (4 * 8 * 5 + 2 * 4 * rb + rb)
+
+    def cost(self, r):
+        return r.n * r.n * self.bytes_per_cell
+
+    def run(self, requests, realtime=True):
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        used = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        finished = []
+        rebuilds = 0
+        while pending or waiting or any(g.reqs for g in self.groups.values()):
+            tn = now()
+            while pending and (pending[0].arrival <= tn or not realtime):
+                waiting.append(pending.popleft())
+            while waiting and used + self.cost(waiting[0]) <= self.budget:
+                r = waiting.popleft()
+                used += self.cost(r)
+                r.admitted_at = now()
+                g = self.groups.get(r.subgrid)
+                if g is None:
+                    g = self.groups[r.subgrid] = _Group(r.subgrid, self.precision, self.compat, self.cfl, self.device)
+                g.reqs.append(r)
+                g.stale = True
+            for g in self.groups.values():
+                if g.stale:
+                    g.rebuild()
+                    rebuilds += 1
+            if not any(g.reqs for g in self.groups.values()):
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                continue
+            for g in self.groups.values():
+                g.advance(self.chunk)
+            for g in self.groups.values():
+                for r in g.harvest(now, self.keep_results):
+                    used -= self.cost(r)
+                    finished.append(r)
+        for g in self.groups.values():
+            if g.plan:
+                g.plan.close()
+        return _summary(finished, now(), rebuilds)
+
+
+def run_fifo(requests, precision="fp64", compat="physical", cfl=.4, chunk=64, realtime=True, keep_results=False,
+             device="cuda:0"):
+    """The plain loop: one request at a time in arrival order."""
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0
+    done = []
+    for r in sorted(requests, key=lambda r: r.arrival):
+        if realtime and r.arrival > now():
+            time.sleep(r.arrival - now())
+        r.admitted_at = now()
+        plan = Plan(2, r.subgrid, "hllc", "ssprk(3,3)", [_problem(r, cfl)], precision=precision, varmask=0x17,
+                    compat=compat, device=device)
+        plan.load([r.grid])
+        plan.reset()
+        r.status = plan.run(chunk)[0]
+        r.done_at = now()
+        if keep_results:
+            r.result = plan.state.clone()
+        plan.close()
+        done.append(r)
+    return _summary(done, now(), 0)
+
+
+def _summary(done, makespan, rebuilds):
+    import numpy as np
+    lat = np.array([r.latency for r in done]) if done else np.zeros(0)
+    zu = sum(r.n * r.n * r.status["steps"] for r in done)
+    first = min((r.arrival for r in done), default=0.0)
+    return dict(requests=len(done), failed=sum(bool(r.status["failed"]) for r in done), zu=zu,
+                seconds=makespan - first, zu_per_s=zu / max(makespan - first, 1e-9),
+                p50=float(np.percentile(lat, 50)) if len(lat) else None,
+                p99=float(np.percentile(lat, 99)) if len(lat) else None, rebuilds=rebuilds, done=done)

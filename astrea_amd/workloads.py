@@ -1,0 +1,109 @@
+"""Benchmark workloads (4 and 5) as reusable runners.
+
+Each runner builds its synthetic input on the device from a stated seed,
+runs the device update and returns a dict of measurements.  bench.py prints
+them; tests use small instances.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import batch, ic
+from .plan import Plan
+
+SEED = 20261202
+
+
+def rel_l1(a, b):
+    """Max over variables of the relative L1 error of a against b (packed
+    state planes, or reference-layout arrays)."""
+    if isinstance(a, torch.Tensor):
+        num = (a - b).abs().sum(dim=1)
+        den = b.abs().sum(dim=1)
+        return float(torch.max(torch.where(den > 0, num / den.clamp_min(1e-300), num)).item())
+    ax = tuple(range(a.ndim - 1))
+    num = np.abs(a - b).sum(ax)
+    den = np.abs(b).sum(ax)
+    return float(np.max(np.where(den > 0, num / np.where(den > 0, den, 1), num)))
+
+
+# ---------------------------------------------------------------------------
+# Config 4: ragged 1D Riemann ensemble
+# ---------------------------------------------------------------------------
+def ensemble_inputs(nprob=512, seed=SEED, nmin=256, nmax=8192, t_scale=1.0):
+    specs = ic.ensemble_1d(nprob, seed, nmin, nmax)
+    grids, probs = [], []
+    for sp in specs:
+        w = ic.riemann_1d(sp["n"], sp["left"], sp["right"])
+        grids.append(ic.to_device(w, 1.4, "edge", True))
+        probs.append(dict(n0=sp["n"], boundary="edge", dx=1 / sp["n"], gamma=1.4, cfl=.5,
+                          t_end=sp["t_end"] * t_scale))
+    return grids, probs
+
+
+def run_ensemble_batched(grids, probs, precision="fp64", chunk=16, use_graph=True, blocks=None):
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision)
+    if blocks:
+        plan.set_blocks(blocks)
+    plan.load(grids)
+    plan.reset()
+    torch.cuda.synchronize()
+    st, secs, nch = batch.run(plan, chunk=chunk, use_graph=use_graph)
+    zu = sum(p["n0"] * s["steps"] for p, s in zip(probs, st))
+    out = plan.state.clone()
+    offs = plan.offsets
+    plan.close()
+    return dict(seconds=secs, zu=zu, zu_per_s=zu / secs, chunks=nch, status=st, state=out, offsets=offs)
+
+
+def run_ensemble_plain(grids, probs, sample=None, chunk=64):
+    """The plain path: one problem at a time, FP64, eager launches, host
+    looks at the device once per chunk.  ``sample``: indices to run (the rate
+    is what is compared; a stratified sample keeps the run bounded)."""
+    idx = range(len(probs)) if sample is None else sample
+    zu, secs, st = 0, 0.0, {}
+    for i in idx:
+        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", [probs[i]], varmask=0x13)
+        plan.load([grids[i]])
+        plan.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = plan.run(chunk)[0]
+        torch.cuda.synchronize()
+        secs += time.perf_counter() - t0
+        zu += probs[i]["n0"] * s["steps"]
+        st[i] = s
+        plan.close()
+    return dict(seconds=secs, zu=zu, zu_per_s=zu / secs, status=st)
+
+
+def autotune_blocks(grids, probs, precision, candidates=(512, 1024, 2048, 4096), steps=64):
+    """Pick the persistent grid cap on the problem set itself: time a
+    fixed number of steps for each candidate (state restored between)."""
+    best, times = None, {}
+    for b in candidates:
+        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision)
+        plan.set_blocks(b)
+        plan.load(grids)
+        plan.reset()
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            plan.step_graph(16, 1)
+            s.synchronize()
+            t0 = time.perf_counter()
+            plan.step_graph(16, steps // 16)
+            s.synchronize()
+            times[b] = time.perf_counter() - t0
+        plan.close()
+        if best is None or times[b] < times[best]:
+            best = b
+    return best, times
+
+
+# ---------------------------------------------------------------------------
+# Config 2: 2D shear layer (see bench.py)
+# ---------------------------------------------------------------------------
+def shear_inputs(n, seed=SEED):
+    return ic.to_device(ic.shear_layer(n, seed), 1.4, "wrap", True)

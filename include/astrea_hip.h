@@ -105,6 +105,16 @@ typedef struct ah_problem_status {
     int failed;        /* 1 once a non-finite Jacobian state appeared (the reference's LAPACK call raises there) */
 } ah_problem_status;
 
+/* Full per-problem time state (what a problem carries between steps). */
+typedef struct ah_time_state {
+    double t, dt;
+    long long steps;
+    int idx;      /* next checkpoint index (astrea.py:44,78) */
+    int active;
+    int failed;
+    int parity;   /* sweep-order parity (astrea.py:85) */
+} ah_time_state;
+
 typedef struct ah_plan ah_plan;
 
 /* Library identity. */
@@ -150,6 +160,13 @@ int ah_plan_compact(ah_plan* plan, int* nactive, void* stream);
 
 /* Persistent grid cap: workgroups per launch (each loops over its tiles). */
 int ah_plan_set_blocks(ah_plan* plan, int max_blocks);
+
+/* Export / import the time state of all problems of a plan (synchronous),
+ * so a problem can move between plans with its state planes: the serving
+ * loop re-plans its batch as requests arrive and finish.  No reference
+ * counterpart (the reference runs one problem per process). */
+int ah_plan_time_export(ah_plan* plan, ah_time_state* out, void* stream);
+int ah_plan_time_import(ah_plan* plan, const ah_time_state* in, void* stream);
 
 /* Copy the per-problem time state to the host (synchronises the stream). */
 int ah_plan_status(ah_plan* plan, ah_problem_status* out, void* stream);

@@ -274,3 +274,29 @@ def test_batch_vs_oracle():
             continue
         assert st[i]["steps"] == ref.steps
         assert np.max(rel_l1(outs[i], ref.grid)) <= 1e-12
+
+
+def test_served_results_equal_alone():
+    """Continuous batching: requests of two schemes and three sizes admitted
+    at different times into re-planned batches give, request by request, the
+    same FP64 state (bitwise), step count and failure status as running each
+    request alone."""
+    from astrea_amd import serve
+    reqs = serve.make_requests(8, 3, rate=50.0, sizes=(24, 32, 48))
+    for r in reqs:
+        r.t_end = .03
+    _say(f"[gpu] serve equivalence: {[(r.n, r.subgrid) for r in reqs]}")
+    srv = serve.Server(hbm_budget=3 * 48 * 48 * 232, keep_results=True, chunk=4)
+    mix = srv.run(reqs, realtime=False)
+    res_mix = {r.rid: (r.result.clone(), dict(r.status)) for r in mix["done"]}
+    assert mix["rebuilds"] > 2
+    alone = serve.run_fifo(reqs, realtime=False, keep_results=True)
+    for r in alone["done"]:
+        got, st = res_mix[r.rid]
+        assert st["steps"] == r.status["steps"] and st["failed"] == r.status["failed"], (r.rid, st, r.status)
+        assert torch_equal(got, r.result), r.rid
+
+
+def torch_equal(a, b):
+    import torch
+    return bool(torch.equal(a, b))
