@@ -394,10 +394,9 @@ __device__ inline V8<R> flux_llf(const Iface<R>& s, R lam) {
 
 // HLLC (solvers.py:92-138).  sax = the solver's axis argument (the
 // enumeration index of the sweep, solvers.py:34-36); the reference writes the
-// star momentum at index 1 whatever the axis (127-128), `mom` selects it;
-// F(w+) is the default (133).
+// star momentum at index 1 whatever the axis (127-128), `mom` selects it.
 template <class R, int MASK>
-__device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma, int mom) {
+__device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma, int mom, bool default_minus) {
     using V = VS<MASK>;
     const R rhoL = s.wm.c[0], pL = s.wm.c[4], rhoR = s.wp.c[0], pR = s.wp.c[4];
     const R uL = sax == 0 ? s.wm.c[1] : (V::live(2) ? s.wm.c[2] : R(0));
@@ -415,9 +414,12 @@ __device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma, int mom) 
     const bool useL = (sL <= R(0)) && (R(0) < sM);
     const bool useR = (sM <= R(0)) && (R(0) <= sR);
     const bool useP = sR < R(0);
-    // select first, then evaluate one star flux: F(w+) | F*L | F*R | F(w+)
+    // select first, then evaluate one star flux: F(w+) | F*L | F*R | default.
+    // The reference's default is F(w+) (solvers.py:133), which is the
+    // downwind flux when all waves move right; `default_minus` gives F(w-).
+    if (useP) return s.fp;
+    if (!(useL || useR)) return default_minus ? s.fm : s.fp;
     V8<R> F = s.fp;
-    if (useP || !(useL || useR)) return F;
     const bool left = useL && !useR;
     const V8<R>& Q = left ? s.qm : s.qp;
     const V8<R>& Fb = left ? s.fm : s.fp;

@@ -228,11 +228,11 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
 }
 
 template <class R, int MASK, int SOLVER, int AX>
-__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma, int mom = 1) {
+__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma, int mom, bool physical) {
     if constexpr (SOLVER == LLF)
         return flux_llf<R, MASK>(s, lam);
     else if constexpr (SOLVER == HLLC)
-        return flux_hllc<R, MASK>(s, sax, gamma, mom);
+        return flux_hllc<R, MASK>(s, sax, gamma, physical ? mom : 1, physical);
     else
         return sax == 0 ? flux_hlld<R, MASK, 0>(s, Bn, gamma) : flux_hlld<R, MASK, 1>(s, Bn, gamma);
 }
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         const R lam = nmax(rad[i], rad[i + 1]);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = W(e(k)).c[5];
-        const V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR);
+        const V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR, 1, a.compat != 0);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) fl[V::slot(c)][i] = F.c[c];
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             bn_s[ia][ik] = Bn;
         }
         const R lam = nmax(rad_s[ia][ik], rad_s[ia][ik + 1]);
-        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, a.compat ? sax + 1 : 1);
+        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) {
@@ -806,7 +806,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
         const R lam = nmax(rad_s[ia0][ik], rad_s[ia0][ik + 1]);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = bn_s[ia0][ik];
-        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, a.compat ? sax + 1 : 1);
+        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0);
         const int k = s0 + ik;
 #pragma unroll
         for (int c = 0; c < 8; ++c)

@@ -43,8 +43,8 @@ def ensemble_inputs(nprob=512, seed=SEED, nmin=256, nmax=8192, t_scale=1.0):
     return grids, probs
 
 
-def run_ensemble_batched(grids, probs, precision="fp64", chunk=16, use_graph=True, blocks=None):
-    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision)
+def run_ensemble_batched(grids, probs, precision="fp64", chunk=16, use_graph=True, blocks=None, compat="physical"):
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
     if blocks:
         plan.set_blocks(blocks)
     plan.load(grids)
@@ -58,14 +58,14 @@ def run_ensemble_batched(grids, probs, precision="fp64", chunk=16, use_graph=Tru
     return dict(seconds=secs, zu=zu, zu_per_s=zu / secs, chunks=nch, status=st, state=out, offsets=offs)
 
 
-def run_ensemble_plain(grids, probs, sample=None, chunk=64):
+def run_ensemble_plain(grids, probs, sample=None, chunk=64, compat="physical"):
     """The plain path: one problem at a time, FP64, eager launches, host
     looks at the device once per chunk.  ``sample``: indices to run (the rate
     is what is compared; a stratified sample keeps the run bounded)."""
     idx = range(len(probs)) if sample is None else sample
     zu, secs, st = 0, 0.0, {}
     for i in idx:
-        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", [probs[i]], varmask=0x13)
+        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", [probs[i]], varmask=0x13, compat=compat)
         plan.load([grids[i]])
         plan.reset()
         torch.cuda.synchronize()
@@ -79,12 +79,12 @@ def run_ensemble_plain(grids, probs, sample=None, chunk=64):
     return dict(seconds=secs, zu=zu, zu_per_s=zu / secs, status=st)
 
 
-def autotune_blocks(grids, probs, precision, candidates=(512, 1024, 2048, 4096), steps=64):
+def autotune_blocks(grids, probs, precision, candidates=(512, 1024, 2048, 4096), steps=64, compat="physical"):
     """Pick the persistent grid cap on the problem set itself: time a
     fixed number of steps for each candidate (state restored between)."""
     best, times = None, {}
     for b in candidates:
-        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision)
+        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
         plan.set_blocks(b)
         plan.load(grids)
         plan.reset()

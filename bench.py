@@ -103,6 +103,7 @@ def bench_ensemble(args):
     res = {}
     for prec in ("fp64", "split"):
         res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, chunk=16, blocks=blocks)
+    literal = W.run_ensemble_batched(grids, probs, precision="fp64", chunk=16, blocks=blocks, compat="astrea")
     ref = res["fp64"]
     errs = []
     for i in range(len(probs)):
@@ -121,11 +122,14 @@ def bench_ensemble(args):
         "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if best == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"{args.nprob} ragged 1D Riemann problems, N log-uniform [256,8192], PPM/HLLC/"
-                               "SSPRK(3,3), CFL 0.5, each to its own t_end, one batched plan", "path": best,
+                               "SSPRK(3,3), CFL 0.5, each to its own t_end, one batched plan, HLLC with the physical axis and "
+                               "upwind default", "path": best,
                    "seed": SEED, "cells": sum(p["n0"] for p in probs)},
         "batched": {p: {"zu_per_s": r["zu_per_s"], "seconds": r["seconds"], "graph_chunks": r["chunks"],
                         "failed": sum(s["failed"] for s in r["status"])} for p, r in res.items()},
         "plain_fp64_sample": {"zu_per_s": plain["zu_per_s"], "problems": len(sample), "seconds": plain["seconds"]},
+        "reference_literal_hllc_fp64": {"zu_per_s": literal["zu_per_s"],
+                                        "failed": sum(s["failed"] for s in literal["status"])},
         "speedup_vs_plain": value / plain["zu_per_s"],
         "split_l1_vs_fp64": {"max": max(errs) if errs else None, "median": float(np.median(errs)) if errs else None,
                              "p90": float(np.percentile(errs, 90)) if errs else None},

@@ -60,8 +60,10 @@ enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };
  * the enumeration index of the sweep, which swaps the normal velocity of
  * HLLC/HLLD between the axes on every other step (solvers.py:34-36,63,
  * astrea.py:85), with the HLLC star momentum always at index 1
- * (solvers.py:127-128).  AH_COMPAT_PHYSICAL passes the physical sweep axis
- * and writes the star momentum at that axis. */
+ * (solvers.py:127-128) and F(w+) as the flux when no HLLC region mask applies,
+ * i.e. when all waves move right (solvers.py:133).  AH_COMPAT_PHYSICAL passes
+ * the physical sweep axis, writes the star momentum at that axis and returns
+ * the upwind F(w-) there. */
 enum ah_compat { AH_COMPAT_ASTREA = 0, AH_COMPAT_PHYSICAL = 1 };  /* SPLIT: FP32 reconstruction/limiter/Riemann, FP64 state */
 enum ah_varmask {
     AH_VARS_H1 = 0x13,  /* rho, rho*vx, E          : 1D hydrodynamics */

@@ -546,7 +546,7 @@ def flux_lw(chars, d):
     return .5 * (fm + fp) - .5 * ((qp - qm) * mx[..., None])
 
 
-def flux_hllc(ax, gamma, d, mom=1):
+def flux_hllc(ax, gamma, d, mom=1, default_minus=False):
     """solvers.py:92-138 with ``ax`` the solver's axis argument (the
     enumeration index of the sweep, solvers.py:34-36).  Keeps the reference's
     star momentum at index 1 (127-128) and the F(w+) default (133)."""
@@ -568,7 +568,7 @@ def flux_hllc(ax, gamma, d, mom=1):
     QRs[..., mom] = rhoR * kR * sM
     QLs[..., 4] = QLs[..., 4] + kL * (sM - uL) * (rhoL * sM + gdiv(pL, sL - uL))
     QRs[..., 4] = QRs[..., 4] + kR * (sM - uR) * (rhoR * sM + gdiv(pR, sR - uR))
-    F = np.copy(fp)
+    F = np.copy(fm if default_minus else fp)
     FLs, FRs = fm + (QLs - QL) * sL[..., None], fp + (QRs - QR) * sR[..., None]
     m = (sL <= 0) & (0 < sM)
     F[m] = FLs[m]
@@ -695,7 +695,9 @@ def riemann(s, ax, lmax, chars, d):
     if s.solver_category == "hll":
         if s.solver.endswith("d"):
             return flux_hlld(ax, s.gamma, s.boundary, d)
-        return flux_hllc(ax, s.gamma, d, 1 if s.compat == "astrea" else ax + 1)
+        if s.compat == "astrea":
+            return flux_hllc(ax, s.gamma, d)
+        return flux_hllc(ax, s.gamma, d, ax + 1, default_minus=True)
     if s.solver_category == "complete":
         return flux_es(ax, s.gamma, d)
     if s.solver.endswith("w"):

@@ -254,21 +254,24 @@ def test_batch_equals_alone(use_graph):
     _say(f"[gpu]   {nch} chunks, failed={sum(s['failed'] for s in st)}")
 
 
-def test_batch_vs_oracle():
-    """Two problems of the batch against the oracle."""
+@pytest.mark.parametrize("compat", ["astrea", "physical"])
+def test_batch_vs_oracle(compat):
+    """Every non-failing problem of a batch against the oracle, in both
+    compatibility modes."""
     from astrea_amd.plan import Plan
     from astrea_amd import batch
     grids, probs = _ensemble(6, 5, 32, 128)
-    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13)
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, compat=compat)
     plan.load(grids)
     plan.reset()
-    _say("[gpu] batch vs oracle")
+    _say(f"[gpu] batch vs oracle, compat={compat}")
     st, _, _ = batch.run(plan, chunk=8)
     outs = plan.unpack()
     plan.close()
     for i in range(len(grids)):
         p = probs[i]
-        s = O.Scheme(dimension=1, subgrid="ppm", solver="hllc", boundary="edge", gamma=1.4, cfl=.5, dx=p["dx"])
+        s = O.Scheme(dimension=1, subgrid="ppm", solver="hllc", boundary="edge", gamma=1.4, cfl=.5, dx=p["dx"],
+                     compat=compat)
         ref = O.run(grids[i], s, p["t_end"])
         if st[i]["failed"]:
             continue
@@ -300,3 +303,29 @@ def test_served_results_equal_alone():
 def torch_equal(a, b):
     import torch
     return bool(torch.equal(a, b))
+
+
+def test_quadrants_physical_compat():
+    """Seeded four-quadrant requests (PLM and PPM, edge boundaries) with the
+    physical HLLC run to t_end on the device and equal the oracle."""
+    from astrea_amd import ic
+    from astrea_amd.plan import Plan
+    import torch
+    for sp in ic.quadrant_requests(4, 20261202, sizes=(24,)):
+        n = sp["n"]
+        q0 = ic.initial_average(torch.from_numpy(ic.quadrants(n, sp["states"])), 1.4, "edge",
+                                sp["subgrid"] == "ppm").numpy()
+        _say(f"[gpu] quadrants {n}^2 {sp['subgrid']}")
+        plan = Plan(2, sp["subgrid"], "hllc", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="edge", dx=1 / n, gamma=1.4,
+                                                                    cfl=.4, t_end=.25)], varmask=0x17,
+                    compat="physical")
+        plan.load([q0])
+        plan.reset()
+        st = plan.run(64)[0]
+        out = plan.unpack(0)
+        plan.close()
+        s = O.Scheme(dimension=2, subgrid=sp["subgrid"], solver="hllc", boundary="edge", gamma=1.4, cfl=.4, dx=1 / n,
+                     compat="physical")
+        ref = O.run(q0, s, .25)
+        assert not st["failed"] and st["steps"] == ref.steps
+        assert np.max(rel_l1(out, ref.grid)) <= 1e-12
