@@ -80,8 +80,9 @@ struct ah_plan {
     // device tables
     ProbDesc* d_pd = nullptr;
     ProbState* d_ps = nullptr;
-    // tile lists: 0 = 1D sweep, 1/2 = recon ax0/ax1, 3/4 = flux ax0/ax1, 5 = cells (combine)
-    TileList tl[6];
+    // tile lists: 0 = 1D sweep, 1/2 = recon ax0/ax1, 3/4 = flux ax0/ax1, 5 = cells (combine),
+    // 6/7 = transverse reconstruction ax0/ax1 (MHD)
+    TileList tl[8];
     int max_blocks = 2048;  // persistent grid cap per launch
     // captured step graphs (ah_plan_step_graph)
     hipGraphExec_t graph_exec = nullptr;
@@ -94,6 +95,12 @@ struct ah_plan {
     void* wL = nullptr;
     void* wR = nullptr;
     void* bn = nullptr;
+    // MHD constrained transport buffers
+    void* wF[2] = {nullptr, nullptr};
+    void* wD[2] = {nullptr, nullptr};
+    void* wU[2] = {nullptr, nullptr};
+    double* e3 = nullptr;
+    double* scratch = nullptr;
     bool bound = false;
     // kernel timing (ah_plan_profile)
     std::vector<hipEvent_t> ev;
@@ -140,7 +147,9 @@ const char* ah_status_str(int status) {
 int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_plan** out) {
     if (!sch || !probs || !out || nprob <= 0) return AH_ERR_INVALID;
     if (sch->dims != 1 && sch->dims != 2) return AH_ERR_INVALID;
-    if (sch->magnetic_2d) return AH_ERR_UNSUPPORTED;
+    if (sch->magnetic_2d && (sch->dims != 2 || !(sch->varmask & 0x60) || sch->rk == AH_SSPRK104 ||
+                             sch->flattener != AH_FLAT_OFF || sch->precision != AH_FP64))
+        return AH_ERR_UNSUPPORTED;
     ah_plan* p = new (std::nothrow) ah_plan();
     if (!p) return AH_ERR_NOMEM;
     p->sch = *sch;
@@ -157,6 +166,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
         ok = sch->dims == 1 ? kernels_1d_fp32(sch->varmask, sch->recon, sch->solver, &p->ks)
                             : kernels_2d_fp32(sch->varmask, sch->recon, sch->solver, &p->ks);
     p->prog = rk_program(sch->rk);
+    if (sch->magnetic_2d && !p->ks.corner) ok = false;
     if (!ok || p->prog.empty()) {
         delete p;
         return AH_ERR_UNSUPPORTED;
@@ -205,6 +215,9 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
                     for (int s0 = 0; s0 < ns; s0 += p->ks.rts[ax]) p->tl[1 + ax].full.push_back({i, s0, t0, 0});
                 for (int t0 = 0; t0 < nt; t0 += p->ks.ftt[ax])
                     for (int s0 = 0; s0 < ns; s0 += p->ks.fts[ax]) p->tl[3 + ax].full.push_back({i, s0, t0, 0});
+                if (sch->magnetic_2d)
+                    for (int t0 = 0; t0 < nt; t0 += p->ks.ttt)
+                        for (int s0 = 0; s0 < ns; s0 += p->ks.tts) p->tl[6 + ax].full.push_back({i, s0, t0, 0});
             }
         }
         for (long long c0 = 0; c0 < cells; c0 += 1024)
@@ -278,6 +291,10 @@ int ah_plan_workspace_size(const ah_plan* p, size_t* bytes) {
         b += plane_bytes(p, 8);                        // D
         b += 2 * plane_bytes(p, p->ks.real_bytes);     // wL, wR
         b += ((size_t)p->total * p->ks.real_bytes + 255) & ~size_t(255);  // normal field for HLLD
+        if (p->sch.magnetic_2d) {
+            b += 6 * plane_bytes(p, p->ks.real_bytes);                          // wF, wD, wU per sweep
+            b += 3 * (((size_t)p->total * 8 + 255) & ~size_t(255));            // e3, 2 scratch planes
+        }
     }
     *bytes = b;
     return AH_OK;
@@ -321,6 +338,20 @@ int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
         p->wR = c;
         c += plane_bytes(p, p->ks.real_bytes);
         p->bn = c;
+        c += ((size_t)p->total * p->ks.real_bytes + 255) & ~size_t(255);
+        if (p->sch.magnetic_2d) {
+            for (int ax = 0; ax < 2; ++ax) {
+                p->wF[ax] = c;
+                c += plane_bytes(p, p->ks.real_bytes);
+                p->wD[ax] = c;
+                c += plane_bytes(p, p->ks.real_bytes);
+                p->wU[ax] = c;
+                c += plane_bytes(p, p->ks.real_bytes);
+            }
+            p->e3 = (double*)c;
+            c += ((size_t)p->total * 8 + 255) & ~size_t(255);
+            p->scratch = (double*)c;
+        }
     }
     p->bound = true;
     return AH_OK;
@@ -379,6 +410,7 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
     for (int ax = 0; ax < 2; ++ax) {
         a.tiles = p->tl[1 + ax].d;
         a.ntiles_dev = p->tl[1 + ax].d_count;
+        a.wF = p->sch.magnetic_2d ? p->wF[ax] : nullptr;
         {
             ProfScope ps_(p, ax == 0 ? AH_K_RECON0 : AH_K_RECON1, s);
             p->ks.recon[ax](a, blocks_for(p, p->tl[1 + ax]), s);
@@ -411,17 +443,70 @@ static void launch_combine(ah_plan* p, int op, hipStream_t s) {
     hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(p, p->tl[5])), dim3(256), 0, s, c);
 }
 
+// register each stage combination writes (for the MHD refinement)
+static int target_reg(int op) {
+    switch (op) {
+        case OP_EULER: case OP_22_1: case OP_33_2: case OP_43_3: case OP_RK4_3: case OP_53_4: case OP_54_4: return 0;
+        case OP_22_0: case OP_33_0: case OP_43_0: case OP_RK4_0: case OP_53_0: case OP_54_0: return 1;
+        case OP_33_1: case OP_43_1: case OP_RK4_1: case OP_53_1: case OP_54_1: return 2;
+        case OP_43_2: case OP_RK4_2: case OP_53_2: case OP_54_2: return 3;
+        case OP_53_3: case OP_54_3: return 4;
+    }
+    return -1;
+}
+
+static MhdArgs mhd_args(ah_plan* p) {
+    MhdArgs m;
+    memset(&m, 0, sizeof(m));
+    for (int ax = 0; ax < 2; ++ax) {
+        m.src[ax] = p->wF[ax];
+        m.wD[ax] = p->wD[ax];
+        m.wU[ax] = p->wU[ax];
+        m.ttiles[ax] = p->tl[6 + ax].d;
+        m.nttiles_dev[ax] = p->tl[6 + ax].d_count;
+    }
+    m.e3 = p->e3;
+    m.L = p->Lbuf;
+    m.U = p->U;
+    m.scratch = p->scratch;
+    m.plane = p->total;
+    m.pd = p->d_pd;
+    m.ps = p->d_ps;
+    m.tiles = p->tl[5].d;
+    m.ntiles_dev = p->tl[5].d_count;
+    m.solver_hll = p->sch.solver != AH_LLF;
+    return m;
+}
+
 // one full time step of every active problem
 static void launch_step(ah_plan* p, hipStream_t s) {
     const int pb = (p->nprob + 255) / 256;
+    const bool mhd = p->sch.magnetic_2d != 0;
     for (size_t st = 0; st < p->prog.size(); ++st) {
         const Stage& S = p->prog[st];
         launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s);
+        MhdArgs m;
+        if (mhd) {
+            // corner EMF from both sweeps' face values; it replaces the B rows of L
+            m = mhd_args(p);
+            const int nc = blocks_for(p, p->tl[5]);
+            p->ks.trecon[0](m, blocks_for(p, p->tl[6]), s);
+            p->ks.trecon[1](m, blocks_for(p, p->tl[7]), s);
+            p->ks.corner(m, nc, s);
+            p->ks.magL(m, nc, s);
+        }
         if (st == 0) {
             ProfScope ps_(p, AH_K_DT, s);
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
+        if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[5]), s);
         launch_combine(p, S.op, s);
+        if (mhd) {
+            double* K = p->regs[target_reg(S.op)];
+            m.K = K;
+            p->ks.refine(m, blocks_for(p, p->tl[5]), s);
+            p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
+        }
     }
     ProfScope ps_(p, AH_K_ADVANCE, s);
     hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);

@@ -40,6 +40,32 @@ inline void set2d(KernelSet* ks) {
     ks->fts[0] = ks->fts[1] = FTS;
     ks->ftt[0] = ks->ftt[1] = FTT;
     ks->real_bytes = sizeof(R);
+    if constexpr (VS<MASK>::MHD) {
+        constexpr int TTS = 64, TTT = 8;
+        ks->tts = TTS;
+        ks->ttt = TTT;
+        ks->trecon[0] = [](const MhdArgs& a, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((trecon_kernel<R, MASK, 0, TTS, TTT>), dim3(nb), dim3(256), 0, s, a);
+        };
+        ks->trecon[1] = [](const MhdArgs& a, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((trecon_kernel<R, MASK, 1, TTS, TTT>), dim3(nb), dim3(256), 0, s, a);
+        };
+        ks->corner = [](const MhdArgs& a, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((corner_kernel<R, MASK>), dim3(nb), dim3(256), 0, s, a);
+        };
+        ks->magL = [](const MhdArgs& a, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((magL_kernel<MASK>), dim3(nb), dim3(256), 0, s, a);
+        };
+        ks->bface = [](const MhdArgs& a, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((bface_kernel<R, MASK>), dim3(nb), dim3(256), 0, s, a);
+        };
+        ks->refine = [](const MhdArgs& a, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((refine_kernel<MASK>), dim3(nb), dim3(256), 0, s, a);
+        };
+        ks->copyB = [](const MhdArgs& a, double* dst, int nb, hipStream_t s) {
+            if (nb > 0) hipLaunchKernelGGL((copyB_kernel<MASK>), dim3(nb), dim3(256), 0, s, a, dst);
+        };
+    }
 }
 
 template <class R, int MASK, int SOLVER, int DIMS>

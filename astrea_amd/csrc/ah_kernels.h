@@ -40,6 +40,7 @@ struct SweepArgs {
     void* wL;             // 2D interface states, sweep-frame packed SoA, type R
     void* wR;
     void* bn;             // 2D HLLD: centred normal field per cell (type R)
+    void* wF;             // 2D MHD: the sweep's face values feeding the corner EMF (type R)
     double* flux_out;     // optional: interface fluxes (parity entry point)
     long long plane;
     long long flux_plane;
@@ -589,6 +590,22 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
                 wRo[V::slot(c) * P + o] = Rr.c[c];
             }
         if constexpr (V::MHD) {
+            // face values of this sweep for the constrained transport: the
+            // unlimited interpolant for PPM (ppm.py:38,101), wR otherwise
+            // (plm.py:97, weno.py:186; wR = wS for PCM, pcm.py:34)
+            if (a.wF) {
+                V8<R> f = Rr;
+                if constexpr (RECON == PPM) {
+                    const V8<R> am = W(s - 1), a0 = W(s), a1 = W(s + 1), a2 = W(s + 2);
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (V::live(c)) f.c[c] = R(K7_12) * (a0.c[c] + a1.c[c]) - R(K1_12) * (am.c[c] + a2.c[c]);
+                }
+                R* wFo = reinterpret_cast<R*>(a.wF);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) wFo[V::slot(c) * P + o] = f.c[c];
+            }
             // HLLD's normal field: component (solver axis + 5) of the centred
             // state, the solver axis being the sweep's enumeration index
             // (solvers.py:160,168,183)

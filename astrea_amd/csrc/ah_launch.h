@@ -3,11 +3,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "ah_kernels.h"
+#include "ah_mhd.h"
 
 namespace ah {
 
 using SweepLaunch = void (*)(const SweepArgs&, int nblocks, hipStream_t);
+using MhdLaunch = void (*)(const MhdArgs&, int nblocks, hipStream_t);
+using CopyLaunch = void (*)(const MhdArgs&, double* dst, int nblocks, hipStream_t);
 
 // Tile shapes are compile-time constants of each instantiation; the host
 // builds tile tables from them.
@@ -19,6 +21,11 @@ struct KernelSet {
     int rts[2] = {0, 0}, rtt[2] = {0, 0};
     int fts[2] = {0, 0}, ftt[2] = {0, 0};
     int real_bytes = 8;
+    // 2D MHD constrained transport
+    MhdLaunch trecon[2] = {nullptr, nullptr};
+    MhdLaunch corner = nullptr, magL = nullptr, bface = nullptr, refine = nullptr;
+    CopyLaunch copyB = nullptr;
+    int tts = 64, ttt = 8;
 };
 
 // dims: 1/2; mask: AH_VARS_*; recon: PCM..WENO7; solver: LLF/HLLC/HLLD.

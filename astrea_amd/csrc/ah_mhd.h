@@ -1,0 +1,290 @@
+// 2D MHD constrained transport on gfx950 (num_methods/mag_field.py,
+// num_methods/evolvers.py:26-32, 51-58, 63-67, 73-76).
+//
+// Frames: per sweep AX the interface-state arrays are in the sweep-frame
+// layout [t][s] (s along AX).  The reference's "ortho frame" of a sweep is the
+// transpose of its sweep frame, i.e. exactly this [t][s] order, so the
+// transverse reconstruction runs along t of these arrays.  The corner EMF
+// e3U is stored in the frame (x, y) = (i0, i1) of the state.
+#pragma once
+#include "ah_kernels.h"
+
+namespace ah {
+
+struct MhdArgs {
+    const void* src[2];   // per sweep: face values feeding the corners (R, [t][s])
+    void* wD[2];          // per sweep: transverse extrapolants (R, [t][s])
+    void* wU[2];
+    double* e3;           // corner EMF, one plane, frame (x, y)
+    double* L;            // stage L (B components overwritten)
+    double* U;            // state (stage-0 face-field reassignment)
+    const double* K;      // register to refine (B planes)
+    double* scratch;      // two planes for the refined B
+    long long plane;
+    const ProbDesc* pd;
+    const ProbState* ps;
+    const Tile* tiles;    // cell tiles (problem, first cell, count)
+    const int* ntiles_dev;
+    const Tile* ttiles[2];  // transverse-reconstruction tiles per sweep
+    const int* nttiles_dev[2];
+    int solver_hll;         // corner speeds: HLL branch (mag_field.py:136-149) or eigenvalues (152-159)
+};
+
+// PPM 'mc' along t of a sweep's face values (mag_field.py:11-76, method 'ppm')
+template <class R, int MASK, int AX, int TS, int TT>
+__global__ __launch_bounds__(256) void trecon_kernel(MhdArgs a) {
+    using V = VS<MASK>;
+    constexpr int NV = V::NV;
+    constexpr int H = 4, TW = TT + 2 * H;
+    __shared__ R win[NV][TW][TS];
+    const int ntiles = *a.nttiles_dev[AX];
+    const R* src = reinterpret_cast<const R*>(a.src[AX]);
+    R* oD = reinterpret_cast<R*>(a.wD[AX]);
+    R* oU = reinterpret_cast<R*>(a.wU[AX]);
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tile = a.ttiles[AX][ti];
+        const ProbDesc pd = a.pd[tile.prob];
+        if (!a.ps[tile.prob].active) continue;
+        const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
+        const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
+        const long long off = pd.off, P = a.plane;
+        for (int i = threadIdx.x; i < TW * TS; i += blockDim.x) {
+            const int it = i / TS, is = i % TS;
+            const int s = s0 + is;
+            if (s >= ns) continue;
+            const int gt = bcmap(t0 - H + it, nt, bc);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) win[v][it][is] = src[v * P + off + (long long)gt * ns + s];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
+            const int it = i / TS, is = i % TS;
+            const int s = s0 + is, t = t0 + it;
+            if (s >= ns || t >= nt) continue;
+            struct WAcc {
+                R (*w)[TW][TS];
+                int is, bt, n, bc;
+                __device__ int e(int p) const { return bc == EDGE ? clampi(p, 0, n - 1) : p; }
+                __device__ V8<R> operator()(int m) const {
+                    const int j = e(m) - bt;
+                    V8<R> r = zero8<R, MASK>();
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (VS<MASK>::live(c)) r.c[c] = w[VS<MASK>::slot(c)][j][is];
+                    return r;
+                }
+            } W{win, is, t0 - H, nt, bc};
+            V8<R> D, Uu;
+            recon_cell<R, MASK, PPM, AX>(W, t, MINMOD, FLAT_OFF, D, Uu);
+            const long long o = off + (long long)t * ns + s;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) {
+                    oD[V::slot(c) * P + o] = D.c[c];
+                    oU[V::slot(c) * P + o] = Uu.c[c];
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// corner speeds from the Roe average of (plus = wD[m+1], minus = wU[m])
+template <class R, int MASK>
+__device__ inline void corner_speeds(const V8<R>& plus, const V8<R>& minus, int ax, R gamma, bool hll, R& ap, R& am) {
+    const V8<R> avg = roe_avg<R, MASK>(plus, minus);
+    const R rho = avg.c[0], p = avg.c[4];
+    const R vx = ax == 0 ? avg.c[1] : avg.c[2];
+    const R Bx = ax == 0 ? avg.c[5] : avg.c[6];
+    R cf;
+    if (hll) {
+        // mag_field.py:138-145
+        const R cs = ah_sqrt(gamma * gdiv(p, rho));
+        const R ca = gdiv(ah_sqrt(sumsq3<R, MASK, 5>(avg)), ah_sqrt(rho));
+        const R cax = gdiv(Bx, ah_sqrt(rho));
+        const R cs2 = cs * cs, ca2 = ca * ca;
+        cf = ah_sqrt(R(0.5) * (cs2 + ca2 + ah_sqrt(sq(cs2 + ca2) - R(4) * cs2 * (cax * cax))));
+    } else {
+        // signed extreme eigenvalues of the primitive Jacobian (mag_field.py:154-159):
+        // the spectrum is {0, u, u+-c_a, u+-c_s, u+-c_f}, so max = max(u + c_f, 0)
+        const R a2 = gamma * p / rho;
+        const R b2 = sumsq3<R, MASK, 5>(avg) / rho;
+        const R bn2 = Bx * Bx / rho;
+        const R s = a2 + b2;
+        const R disc = s * s - R(4) * a2 * bn2;
+        const R cf2 = R(0.5) * (s + ah_sqrt(disc > R(0) ? disc : R(0)));
+        cf = ah_sqrt(cf2 > R(0) ? cf2 : R(0));
+    }
+    ap = nmax(R(0), vx + cf);
+    am = -nmin(R(0), vx - cf);
+}
+
+// corner EMF (mag_field.py:125-187); one thread per corner (i, j)
+template <class R, int MASK>
+__global__ __launch_bounds__(256) void corner_kernel(MhdArgs a) {
+    using V = VS<MASK>;
+    const int ntiles = *a.ntiles_dev;
+    const R* Dx = reinterpret_cast<const R*>(a.wD[0]);
+    const R* Ux = reinterpret_cast<const R*>(a.wU[0]);
+    const R* Dy = reinterpret_cast<const R*>(a.wD[1]);
+    const R* Uy = reinterpret_cast<const R*>(a.wU[1]);
+    const long long P = a.plane;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = a.tiles[ti];
+        const ProbState& st = a.ps[tl.prob];
+        if (!st.active) continue;
+        const ProbDesc pd = a.pd[tl.prob];
+        const int n0 = pd.n0, n1 = pd.n1;
+        const R gamma = R(pd.gamma);
+        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
+            const int cell = tl.s0 + c;
+            const int i = cell / n1, j = cell % n1;
+            auto ld = [&](const R* arr, long long o) {
+                V8<R> w = zero8<R, MASK>();
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (V::live(k)) w.c[k] = arr[V::slot(k) * P + pd.off + o];
+                return w;
+            };
+            // x-sweep arrays are [t = y][s = x], y-sweep arrays [t = x][s = y]
+            const V8<R> D_x = ld(Dx, (long long)j * n0 + i), U_x = ld(Ux, (long long)j * n0 + i);
+            const V8<R> D_y = ld(Dy, (long long)i * n1 + j), U_y = ld(Uy, (long long)i * n1 + j);
+            R spy_p, spy_m, spx_p, spx_m;
+            corner_speeds<R, MASK>(ld(Dx, (long long)bcmap(j + 1, n1, pd.bc) * n0 + i), U_x, 1, gamma, a.solver_hll,
+                                   spy_p, spy_m);
+            corner_speeds<R, MASK>(ld(Dy, (long long)bcmap(i + 1, n0, pd.bc) * n1 + j), U_y, 0, gamma, a.solver_hll,
+                                   spx_p, spx_m);
+            // roles follow the reference's dict order, which flips every step
+            const bool even = st.parity == 0;
+            const V8<R>& north = even ? D_y : D_x;
+            const V8<R>& south = even ? U_y : U_x;
+            const V8<R>& east = even ? D_x : D_y;
+            const V8<R>& west = even ? U_x : U_y;
+            const R apy = even ? spx_p : spy_p, amy = even ? spx_m : spy_m;
+            const R apx = even ? spy_p : spx_p, amx = even ? spy_m : spx_m;
+            const R h = R(0.5);
+            const R NE = h * (west.c[2] + south.c[2]) * south.c[5] - h * (west.c[1] + south.c[1]) * west.c[6];
+            const R NW = h * (east.c[2] + south.c[2]) * south.c[5] - h * (east.c[1] + south.c[1]) * east.c[6];
+            const R SE = h * (west.c[2] + north.c[2]) * north.c[5] - h * (west.c[1] + north.c[1]) * west.c[6];
+            const R SW = h * (east.c[2] + north.c[2]) * north.c[5] - h * (east.c[1] + north.c[1]) * east.c[6];
+            const R e = gdiv(apx * apy * SW + amx * apy * SE + apx * amy * NW + amx * amy * NE, (apx + amx) * (apy + amy)) -
+                        gdiv(apy * amy, apy + amy) * (north.c[5] - south.c[5]) +
+                        gdiv(apx * amx, apx + amx) * (east.c[6] - west.c[6]);
+            a.e3[pd.off + cell] = double(e);
+        }
+    }
+}
+
+// L of Bx, By from EMF differences (evolvers.py:51-58), overriding the
+// hydrodynamic flux difference of those components
+template <int MASK>
+__global__ __launch_bounds__(256) void magL_kernel(MhdArgs a) {
+    using V = VS<MASK>;
+    const int ntiles = *a.ntiles_dev;
+    const long long P = a.plane;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = a.tiles[ti];
+        if (!a.ps[tl.prob].active) continue;
+        const ProbDesc pd = a.pd[tl.prob];
+        const int n0 = pd.n0, n1 = pd.n1;
+        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
+            const int cell = tl.s0 + c;
+            const int i = cell / n1, j = cell % n1;
+            const double* e = a.e3 + pd.off;
+            const double e0 = e[cell];
+            const double dy = e[(long long)i * n1 + bcmap(j + 1, n1, pd.bc)] - e0;
+            const double dxd = e[(long long)bcmap(i + 1, n0, pd.bc) * n1 + j] - e0;
+            a.L[V::slot(5) * P + pd.off + cell] = -(dy / pd.dx);
+            a.L[V::slot(6) * P + pd.off + cell] = dxd / pd.dx;
+        }
+    }
+}
+
+// stage 0: B_x, B_y of the state := the sweeps' face values (evolvers.py:73-76)
+template <class R, int MASK>
+__global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
+    using V = VS<MASK>;
+    const int ntiles = *a.ntiles_dev;
+    const long long P = a.plane;
+    const R* Fx = reinterpret_cast<const R*>(a.src[0]);
+    const R* Fy = reinterpret_cast<const R*>(a.src[1]);
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = a.tiles[ti];
+        const ProbState& st = a.ps[tl.prob];
+        if (!st.active || st.fail_flag) continue;
+        const ProbDesc pd = a.pd[tl.prob];
+        const int n0 = pd.n0, n1 = pd.n1;
+        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
+            const int cell = tl.s0 + c;
+            const int i = cell / n1, j = cell % n1;
+            a.U[V::slot(5) * P + pd.off + cell] = double(Fx[V::slot(5) * P + pd.off + (long long)j * n0 + i]);
+            a.U[V::slot(6) * P + pd.off + cell] = double(Fy[V::slot(6) * P + pd.off + (long long)i * n1 + j]);
+        }
+    }
+}
+
+// inverse reconstruction of B after a stage (mag_field.py:191-210): writes
+// the refined Bx, By of register K into two scratch planes
+template <int MASK>
+__global__ __launch_bounds__(256) void refine_kernel(MhdArgs a) {
+    using V = VS<MASK>;
+    const int ntiles = *a.ntiles_dev;
+    const long long P = a.plane;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = a.tiles[ti];
+        const ProbState& st = a.ps[tl.prob];
+        if (!st.active || st.fail_flag) continue;
+        const ProbDesc pd = a.pd[tl.prob];
+        const int n0 = pd.n0, n1 = pd.n1, bc = pd.bc;
+        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
+            const int cell = tl.s0 + c;
+            const int i = cell / n1, j = cell % n1;
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax) {
+                // sweep frame of axis ax: s along ax, t along the other axis
+                const int ns = ax == 0 ? n0 : n1, nt = ax == 0 ? n1 : n0;
+                const int s = ax == 0 ? i : j, t = ax == 0 ? j : i;
+                const double* B = a.K + V::slot(5 + ax) * P + pd.off;
+                auto Bat = [&](int ss, int tt) {
+                    const int gs = bcmap(ss, ns, bc), gt = bcmap(tt, nt, bc);
+                    return ax == 0 ? B[(long long)gs * n1 + gt] : B[(long long)gt * n1 + gs];
+                };
+                // face-centred values: x - lap_t(x)/24 (fv.py:67-85, 'face')
+                auto fc = [&](int ss, int tt) {
+                    const int gs = bcmap(ss, ns, bc), gt = bcmap(tt, nt, bc);
+                    const double x0 = Bat(gs, gt);
+                    return x0 - K24 * ((Bat(gs, gt + 1) - x0) - (x0 - Bat(gs, gt - 1)));
+                };
+                // face-centred -> cell-centred (mag_field.py:203)
+                auto cc = [&](int ss, int tt) {
+                    const int gs = bcmap(ss, ns, bc), gt = bcmap(tt, nt, bc);
+                    return -1.0 / 16.0 * (fc(gs - 1, gt) + fc(gs + 2, gt)) + 9.0 / 16.0 * (fc(gs, gt) + fc(gs + 1, gt));
+                };
+                // cell-centred -> cell-averaged, both axes of the sweep frame (fv.py:67-85, 'cntr')
+                const double c0 = cc(s, t);
+                double r = c0 + K24 * ((cc(s + 1, t) - c0) - (c0 - cc(s - 1, t)));
+                r = r + K24 * ((cc(s, t + 1) - c0) - (c0 - cc(s, t - 1)));
+                a.scratch[ax * P + pd.off + cell] = r;
+            }
+        }
+    }
+}
+
+template <int MASK>
+__global__ __launch_bounds__(256) void copyB_kernel(MhdArgs a, double* dst) {
+    using V = VS<MASK>;
+    const int ntiles = *a.ntiles_dev;
+    const long long P = a.plane;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = a.tiles[ti];
+        const ProbState& st = a.ps[tl.prob];
+        if (!st.active || st.fail_flag) continue;
+        const ProbDesc pd = a.pd[tl.prob];
+        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
+            const long long o = pd.off + tl.s0 + c;
+            dst[V::slot(5) * P + o] = a.scratch[o];
+            dst[V::slot(6) * P + o] = a.scratch[P + o];
+        }
+    }
+}
+
+}  // namespace ah

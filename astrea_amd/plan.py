@@ -34,7 +34,7 @@ def infer_varmask(grids, dims, magnetic):
 
 class Plan:
     def __init__(self, dims, subgrid, solver, timestep, problems, *, limiter="minmod", flattener="off",
-                 precision="fp64", varmask=L.AH_VARS_H1, compat="astrea", device="cuda:0"):
+                 precision="fp64", varmask=L.AH_VARS_H1, compat="astrea", magnetic_2d=False, device="cuda:0"):
         self.lib = L.load()
         self.dims = dims
         self.device = torch.device(device)
@@ -47,7 +47,7 @@ class Plan:
         sch.rk = O.rk_of(timestep)
         sch.precision = L.AH_SPLIT if precision == "split" else L.AH_FP64
         sch.varmask = varmask
-        sch.magnetic_2d = 0
+        sch.magnetic_2d = 1 if magnetic_2d else 0
         sch.compat = L.AH_COMPAT_PHYSICAL if compat == "physical" else L.AH_COMPAT_ASTREA
         self.scheme = sch
         arr = (L.AhProblem * len(problems))()

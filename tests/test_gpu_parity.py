@@ -329,3 +329,59 @@ def test_quadrants_physical_compat():
         ref = O.run(q0, s, .25)
         assert not st["failed"] and st["steps"] == ref.steps
         assert np.max(rel_l1(out, ref.grid)) <= 1e-12
+
+
+MHD = sorted(k for k, v in meta()["cases"].items() if v.get("magnetic_2d"))
+
+
+@pytest.mark.parametrize("name", MHD)
+def test_mhd_constrained_transport_trajectory(name):
+    """2D Orszag-Tang with the constrained-transport update (transverse
+    reconstruction, corner EMF with the step-parity role flip, EMF-difference
+    rows of L, stage-0 face-field reassignment, inverse reconstruction after
+    every stage): same steps as the reference, relative L1 <= 1e-12 vs the
+    oracle and vs the reference's trajectory."""
+    from astrea_amd.plan import Plan
+    m, g = meta()["cases"][name], load("steps.npz")
+    grid0 = g[name + "_grid0"]
+    n = grid0.shape[0]
+    _say(f"[gpu] mhd trajectory {name} shape={grid0.shape}")
+    plan = Plan(2, m["subgrid"], m["solver"], m["timestep"],
+                [dict(n0=n, n1=n, boundary=m["boundary"], dx=m["dx"], gamma=m["gamma"], cfl=m["cfl"],
+                      t_end=m["t_end"], max_steps=m["max_steps"])], varmask=0x77, magnetic_2d=True)
+    plan.load([grid0])
+    plan.reset()
+    st = plan.run(chunk=4)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid=m["subgrid"], solver=m["solver"], boundary=m["boundary"], gamma=m["gamma"],
+                 cfl=m["cfl"], dx=m["dx"], magnetic_2d=True)
+    ref = O.run(grid0, s, m["t_end"], max_steps=m["max_steps"])
+    assert st["steps"] == m["steps"] == ref.steps and not st["failed"]
+    e_or, e_gold = np.max(rel_l1(out, ref.grid)), np.max(rel_l1(out, g[name + "_final"]))
+    _say(f"[gpu]   rel L1 vs oracle {e_or:.2e}, vs reference {e_gold:.2e}")
+    assert e_or <= 1e-12 and e_gold <= 1e-12
+
+
+def test_mhd_longer_run_vs_oracle():
+    """Orszag-Tang 32^2, WENO5 + HLLD, 12 steps: device vs oracle."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    n = 32
+    q0 = ic.initial_average(torch.from_numpy(ic.orszag_tang(n)), 5 / 3, "wrap", True).numpy()
+    _say("[gpu] mhd OT 32^2 weno5/hlld 12 steps")
+    plan = Plan(2, "weno5", "hlld", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=5 / 3, cfl=.4,
+                                                          t_end=10., max_steps=12)], varmask=0x77, magnetic_2d=True)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=12)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid="weno5", solver="hlld", boundary="wrap", gamma=5 / 3, cfl=.4, dx=1 / n,
+                 magnetic_2d=True)
+    ref = O.run(q0, s, 10., max_steps=12)
+    assert st["steps"] == 12 and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
