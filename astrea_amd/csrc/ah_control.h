@@ -101,13 +101,13 @@ struct CombineArgs {
 
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
     const long long n = a.plane;
-    const int ntiles = *a.ntiles_dev;
+    const int ntiles = wave_uniform(*a.ntiles_dev);
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = a.tiles[ti];
+        const Tile tl = ld_const(a.tiles + ti);
         const ProbState& s = a.ps[tl.prob];
-        if (!s.active || s.fail_flag) continue;
-        const double dt = s.dt;
-        const long long base = a.pd[tl.prob].off + tl.s0;
+        if (!ld_const(&s.active) || ld_const(&s.fail_flag)) continue;
+        const double dt = ld_const(&s.dt);
+        const long long base = ld_const(&a.pd[tl.prob].off) + tl.s0;
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x)
         for (int v = 0; v < a.nv; ++v) {
             const long long j = v * n + base + c;

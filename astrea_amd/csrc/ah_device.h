@@ -436,6 +436,147 @@ __device__ inline V8<R> flux_hllc(const Iface<R>& s, int sax, R gamma, int mom, 
     return F;
 }
 
+// A value the whole workgroup shares, read from lane 0 so that the compiler
+// keeps it (and control flow on it) scalar: loads from memory the kernel also
+// writes are vector loads, which would otherwise make tile loops divergent.
+__device__ inline int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Read-only tables (tile lists, problem descriptors, flags that only other
+// launches write) read through the constant address space: at a uniform
+// address this is a scalar load, which the compiler cannot otherwise prove
+// safe inside a loop that also stores to global memory.
+template <class T>
+__device__ inline T ld_const(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized tables");
+    using CW = const __attribute__((address_space(4))) int;
+    CW* q = (CW*)p;
+    T out;
+    int* o = reinterpret_cast<int*>(&out);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 4); ++i) o[i] = q[i];
+    return out;
+}
+
+// ---- split-precision helpers ---------------------------------------------------
+// Values (states, interface states, fluxes) stay FP64; only coefficients run in
+// FP32.  These variants are not bitwise reference-equal (they use
+// E = p * 1/(gamma-1), squared norms without the sqrt/square round trip, and
+// refined FP32 reciprocals) but are accurate to ~1e-14 relative.
+__device__ inline double rcp_refined(double x) {
+    const float r0 = 1.0f / float(x);
+    double r = double(r0);
+    if (!(r0 != 0.0f) || !ah_isfinite(r0)) return x != 0.0 ? 1.0 / x : 0.0;
+    r = r + r * (1.0 - x * r);  // one Newton step: relative error ~ (2^-24)^2
+    return r;
+}
+
+template <int MASK>
+__device__ inline V8<double> cons_to_prim_split(const V8<double>& q, double gamma) {
+    using V = VS<MASK>;
+    V8<double> w = q;
+    const double r = q.c[0] != 0.0 ? rcp_refined(q.c[0]) : 0.0;
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (V::live(i)) w.c[i] = q.c[i] * r;
+    double kin = q.c[0] * sumsq3<double, MASK, 1>(w);
+    if constexpr (V::MHD) kin = kin + sumsq3<double, MASK, 5>(q);
+    w.c[4] = (gamma - 1.0) * (q.c[4] - 0.5 * kin);
+    return w;
+}
+
+template <int MASK>
+__device__ inline V8<double> prim_to_cons_split(const V8<double>& w, double gamma) {
+    using V = VS<MASK>;
+    V8<double> q = w;
+    double kin = w.c[0] * sumsq3<double, MASK, 1>(w);
+    if constexpr (V::MHD) kin = kin + sumsq3<double, MASK, 5>(w);
+    q.c[4] = w.c[4] * (1.0 / (gamma - 1.0)) + 0.5 * kin;
+#pragma unroll
+    for (int i = 1; i < 4; ++i)
+        if (V::live(i)) q.c[i] = w.c[i] * w.c[0];
+    return q;
+}
+
+template <int MASK, int AX>
+__device__ inline V8<double> phys_flux_split(const V8<double>& w, double gamma) {
+    using V = VS<MASK>;
+    constexpr int A = AX % 3, O = (AX + 1) % 3, C = (AX + 2) % 3;
+    V8<double> f = zero8<double, MASK>();
+    const double rho = w.c[0], u = w.c[AX + 1], p = w.c[4];
+    const double ru = rho * u;
+    const double g1 = gamma / (gamma - 1.0);
+    f.c[0] = ru;
+    f.c[A + 1] = ru * u + p;
+    if (V::live(O + 1)) f.c[O + 1] = ru * w.c[O + 1];
+    if (V::live(C + 1)) f.c[C + 1] = ru * w.c[C + 1];
+    f.c[4] = u * (0.5 * rho * sumsq3<double, MASK, 1>(w) + g1 * p);
+    if constexpr (V::MHD) {
+        const double b2 = sumsq3<double, MASK, 5>(w);
+        const double Ba = w.c[AX + 5];
+        f.c[A + 1] = f.c[A + 1] + 0.5 * b2 - Ba * Ba;
+        if (V::live(O + 1)) f.c[O + 1] = f.c[O + 1] - Ba * w.c[O + 5];
+        if (V::live(C + 1)) f.c[C + 1] = f.c[C + 1] - Ba * w.c[C + 5];
+        f.c[4] = f.c[4] + u * b2 - Ba * dot_vB<double, MASK>(w);
+        if (V::live(O + 5)) f.c[O + 5] = w.c[O + 5] * u - Ba * w.c[O + 1];
+        if (V::live(C + 5)) f.c[C + 5] = w.c[C + 5] * u - Ba * w.c[C + 1];
+    }
+    return f;
+}
+
+// HLLC in perturbation form: wave speeds and star factors in FP32, the flux
+// assembled in FP64 from F +/- s (Q* - Q) with Q* - Q written through
+// (k - 1) = (sM - u) / (sS - sM) so no O(1) cancellation happens in FP32.
+template <int MASK>
+__device__ inline V8<double> flux_hllc_split(const Iface<double>& s, int sax, double gamma, int mom,
+                                             bool default_minus) {
+    using V = VS<MASK>;
+    const double rhoLd = s.wm.c[0], pLd = s.wm.c[4], rhoRd = s.wp.c[0], pRd = s.wp.c[4];
+    const double uLd = sax == 0 ? s.wm.c[1] : (V::live(2) ? s.wm.c[2] : 0.0);
+    const double uRd = sax == 0 ? s.wp.c[1] : (V::live(2) ? s.wp.c[2] : 0.0);
+    const float rhoL = float(rhoLd), rhoR = float(rhoRd), uL = float(uLd), uR = float(uRd);
+    const float pL = float(pLd), pR = float(pRd), g = float(gamma);
+    // jumps taken in FP64 before rounding: every speed below is carried as an
+    // offset from uL or uR, so sM - u has no cancellation in FP32
+    const float du = float(uRd - uLd), dp = float(pRd - pLd);
+    const float cL = ah_sqrt(g * gdiv(pL, rhoL)), cR = ah_sqrt(g * gdiv(pR, rhoR));
+    const float sqL = ah_sqrt(rhoL), sqR = ah_sqrt(rhoR), ssum = sqL + sqR;
+    const float uhL = gdiv(du * sqR, ssum), uhR = -gdiv(du * sqL, ssum);  // u_hat - uL, u_hat - uR
+    const float c2_hat = gdiv(sqL * (cL * cL) + sqR * (cR * cR), ssum) + 0.5f * (du * du) * gdiv(sqL * sqR, ssum * ssum);
+    const float sc = ah_sqrt(c2_hat);
+    const float sLu = nmin(-cL, uhL - sc), sRu = nmax(cR, uhR + sc);  // sL - uL, sR - uR
+    const float sL = uL + sLu, sR = uR + sRu;
+    // sM = (dp + rhoL uL (sL-uL) - rhoR uR (sR-uR)) / (rhoL (sL-uL) - rhoR (sR-uR)), as offsets
+    const float den0 = rhoL * sLu - rhoR * sRu;
+    const float sMuL = gdiv(dp - rhoR * sRu * du, den0), sMuR = gdiv(dp - rhoL * sLu * du, den0);
+    const float sM = uL + sMuL;
+    const bool useL = (sL <= 0.f) && (0.f < sM);
+    const bool useR = (sM <= 0.f) && (0.f <= sR);
+    if (sR < 0.f) return s.fp;
+    if (!(useL || useR)) return default_minus ? s.fm : s.fp;
+    const bool left = useL && !useR;
+    const V8<double>& Q = left ? s.qm : s.qp;
+    const V8<double>& Fb = left ? s.fm : s.fp;
+    const float sS = left ? sL : sR, rho = left ? rhoL : rhoR, p = left ? pL : pR;
+    const float sSu = left ? sLu : sRu, sMu = left ? sMuL : sMuR;  // sS - u, sM - u
+    const float den = sSu - sMu;                                    // sS - sM
+    // k = (sS - u)/(sS - sM) (0 where the denominator vanishes, fv.divide)
+    const float km1 = den != 0.f ? sMu / den : -1.f;
+    const float k = den != 0.f ? sSu / den : 0.f;
+    const float eadd = k * sMu * (rho * sM + gdiv(p, sSu));
+    const double rho_u = left ? rhoLd * uLd : rhoRd * uRd;
+    V8<double> F = Fb;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (!V::live(i)) continue;
+        double dq = Q.c[i] * double(km1);
+        // rho k sM - Q = rho (u + sS (k - 1)) - Q
+        if (i == mom) dq = (left ? rhoLd : rhoRd) * double(sS) * double(km1) + (rho_u - Q.c[i]);
+        if (i == 4) dq = dq + double(eadd);
+        F.c[i] = Fb.c[i] + dq * double(sS);
+    }
+    return F;
+}
+
 // HLLD (solvers.py:142-232), evaluated on all 8 components exactly as the
 // reference writes it (dead components enter as zeros).  SAX = solver axis;
 // Bn = normal field of the right cell's centred state (solvers.py:168,183).

@@ -1,8 +1,8 @@
-// 1D sweep kernels instantiated for float arithmetic.
+// 1D sweep kernels of the split-precision path (FP64 values, FP32 coefficients).
 #include "ah_inst.h"
 
 namespace ah {
 bool kernels_1d_fp32(int mask, int recon, int solver, KernelSet* ks) {
-    return select_kernels<float, 1>(mask, recon, solver, ks);
+    return select_kernels<double, 1, true>(mask, recon, solver, ks);
 }
 }  // namespace ah

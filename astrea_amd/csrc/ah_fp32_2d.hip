@@ -1,8 +1,8 @@
-// 2D sweep kernels instantiated for float arithmetic.
+// 2D sweep kernels of the split-precision path (FP64 values, FP32 coefficients).
 #include "ah_inst.h"
 
 namespace ah {
 bool kernels_2d_fp32(int mask, int recon, int solver, KernelSet* ks) {
-    return select_kernels<float, 2>(mask, recon, solver, ks);
+    return select_kernels<double, 2, true>(mask, recon, solver, ks);
 }
 }  // namespace ah

@@ -5,33 +5,33 @@
 
 namespace ah {
 
-template <class R, int MASK, int RECON, int SOLVER>
+template <class R, int MASK, int RECON, int SOLVER, bool SPLIT>
 inline void set1d(KernelSet* ks) {
     constexpr int TS = VS<MASK>::NV > 4 ? 128 : 256;
     ks->sweep1d = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS>, a, nb, s);
+        launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS, SPLIT>, a, nb, s);
     };
     ks->ts1 = TS;
     ks->real_bytes = sizeof(R);
 }
 
-template <class R, int MASK, int RECON, int SOLVER>
+template <class R, int MASK, int RECON, int SOLVER, bool SPLIT>
 inline void set2d(KernelSet* ks) {
     constexpr int CONV = RECON == PCM ? 0 : (RECON == PLM ? 1 : 2);
     constexpr int NVB = VS<MASK>::NV * (int)sizeof(R);
     constexpr int RTS0 = 32, RTT0 = 8, RTS1 = 64, RTT1 = 4;
     constexpr int FTS = NVB > 32 ? 16 : 32, FTT = 4;
     ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(recon2d_kernel<R, MASK, RECON, 0, RTS0, RTT0>, a, nb, s);
+        launch_sweep(recon2d_kernel<R, MASK, RECON, 0, RTS0, RTT0, SPLIT>, a, nb, s);
     };
     ks->recon[1] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(recon2d_kernel<R, MASK, RECON, 1, RTS1, RTT1>, a, nb, s);
+        launch_sweep(recon2d_kernel<R, MASK, RECON, 1, RTS1, RTT1, SPLIT>, a, nb, s);
     };
     ks->flux[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 0, FTS, FTT>, a, nb, s);
+        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 0, FTS, FTT, SPLIT>, a, nb, s);
     };
     ks->flux[1] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 1, FTS, FTT>, a, nb, s);
+        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 1, FTS, FTT, SPLIT>, a, nb, s);
     };
     ks->rts[0] = RTS0;
     ks->rtt[0] = RTT0;
@@ -40,7 +40,7 @@ inline void set2d(KernelSet* ks) {
     ks->fts[0] = ks->fts[1] = FTS;
     ks->ftt[0] = ks->ftt[1] = FTT;
     ks->real_bytes = sizeof(R);
-    if constexpr (VS<MASK>::MHD) {
+    if constexpr (VS<MASK>::MHD && !SPLIT) {
         constexpr int TTS = 64, TTT = 8;
         ks->tts = TTS;
         ks->ttt = TTT;
@@ -68,13 +68,13 @@ inline void set2d(KernelSet* ks) {
     }
 }
 
-template <class R, int MASK, int SOLVER, int DIMS>
+template <class R, int MASK, int SOLVER, int DIMS, bool SPLIT>
 inline bool by_recon(int recon, KernelSet* ks) {
     switch (recon) {
-#define AH_CASE(RC)                                        \
-    case RC:                                               \
-        if constexpr (DIMS == 1) set1d<R, MASK, RC, SOLVER>(ks); \
-        else set2d<R, MASK, RC, SOLVER>(ks);               \
+#define AH_CASE(RC)                                                     \
+    case RC:                                                            \
+        if constexpr (DIMS == 1) set1d<R, MASK, RC, SOLVER, SPLIT>(ks); \
+        else set2d<R, MASK, RC, SOLVER, SPLIT>(ks);                     \
         return true;
         AH_CASE(PCM)
         AH_CASE(PLM)
@@ -87,26 +87,26 @@ inline bool by_recon(int recon, KernelSet* ks) {
     return false;
 }
 
-template <class R, int DIMS>
+template <class R, int DIMS, bool SPLIT>
 inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
     if constexpr (DIMS == 1) {
         if (mask == 0x13) {
-            if (solver == LLF) return by_recon<R, 0x13, LLF, 1>(recon, ks);
-            if (solver == HLLC) return by_recon<R, 0x13, HLLC, 1>(recon, ks);
+            if (solver == LLF) return by_recon<R, 0x13, LLF, 1, SPLIT>(recon, ks);
+            if (solver == HLLC) return by_recon<R, 0x13, HLLC, 1, SPLIT>(recon, ks);
         }
         if (mask == 0xFF) {
-            if (solver == LLF) return by_recon<R, 0xFF, LLF, 1>(recon, ks);
-            if (solver == HLLC) return by_recon<R, 0xFF, HLLC, 1>(recon, ks);
-            if (solver == HLLD) return by_recon<R, 0xFF, HLLD, 1>(recon, ks);
+            if (solver == LLF) return by_recon<R, 0xFF, LLF, 1, SPLIT>(recon, ks);
+            if (solver == HLLC) return by_recon<R, 0xFF, HLLC, 1, SPLIT>(recon, ks);
+            if (solver == HLLD) return by_recon<R, 0xFF, HLLD, 1, SPLIT>(recon, ks);
         }
     } else {
         if (mask == 0x17) {
-            if (solver == LLF) return by_recon<R, 0x17, LLF, 2>(recon, ks);
-            if (solver == HLLC) return by_recon<R, 0x17, HLLC, 2>(recon, ks);
+            if (solver == LLF) return by_recon<R, 0x17, LLF, 2, SPLIT>(recon, ks);
+            if (solver == HLLC) return by_recon<R, 0x17, HLLC, 2, SPLIT>(recon, ks);
         }
         if (mask == 0x77) {
-            if (solver == LLF) return by_recon<R, 0x77, LLF, 2>(recon, ks);
-            if (solver == HLLD) return by_recon<R, 0x77, HLLD, 2>(recon, ks);
+            if (solver == LLF) return by_recon<R, 0x77, LLF, 2, SPLIT>(recon, ks);
+            if (solver == HLLD) return by_recon<R, 0x77, HLLD, 2, SPLIT>(recon, ks);
         }
     }
     return false;

@@ -99,7 +99,7 @@ __device__ inline V8<D> cast8(const V8<S>& a) {
 // 4th-order cell-average conversion q -> w (functions/fv.py:126-143), FP64:
 // w = C(q - K24 lap0 q - K24 lap1 q) + (K24 lap0 C(q) + K24 lap1 C(q)),
 // C = pointwise cons->prim.  lap_a x = (x[+1] - x[0]) - (x[0] - x[-1]).
-template <int MASK, int DIM>
+template <int MASK, int DIM, bool SPLIT = false>
 __device__ inline V8<double> ho_cell_prim(const V8<double>& q, const V8<double>& qs0m, const V8<double>& qs0p,
                                           const V8<double>& qt0m, const V8<double>& qt0p, const V8<double>& cq,
                                           const V8<double>& cs0m, const V8<double>& cs0p, const V8<double>& ct0m,
@@ -119,7 +119,7 @@ __device__ inline V8<double> ho_cell_prim(const V8<double>& q, const V8<double>&
         }
         w.c[c] = acc;
     }
-    V8<double> r = cons_to_prim<double, MASK>(qq, gamma);
+    V8<double> r = SPLIT ? cons_to_prim_split<MASK>(qq, gamma) : cons_to_prim<double, MASK>(qq, gamma);
 #pragma unroll
     for (int c = 0; c < 8; ++c)
         if (V::live(c)) r.c[c] = r.c[c] + w.c[c];
@@ -228,12 +228,46 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
     }
 }
 
+template <int MASK, bool SPLIT>
+__device__ inline V8<double> c2p(const V8<double>& q, double gamma) {
+    if constexpr (SPLIT) return cons_to_prim_split<MASK>(q, gamma);
+    else return cons_to_prim<double, MASK>(q, gamma);
+}
+template <int MASK, bool SPLIT>
+__device__ inline V8<double> p2c(const V8<double>& w, double gamma) {
+    if constexpr (SPLIT) return prim_to_cons_split<MASK>(w, gamma);
+    else return prim_to_cons<double, MASK>(w, gamma);
+}
+template <int MASK, int AX, bool SPLIT>
+__device__ inline V8<double> pflux(const V8<double>& w, double gamma) {
+    if constexpr (SPLIT) return phys_flux_split<MASK, AX>(w, gamma);
+    else return phys_flux<double, MASK, AX>(w, gamma);
+}
+// spectral radius of a Jacobian state: FP32 in the split path (a coefficient)
+template <int MASK, int AX, bool SPLIT>
+__device__ inline double radius(const V8<double>& st, double gamma) {
+    if constexpr (SPLIT) return double(spectral_radius<float, MASK, AX>(cast8<float, double, MASK>(st), float(gamma)));
+    else return spectral_radius<double, MASK, AX>(st, gamma);
+}
+template <int MASK, bool SPLIT, bool ROE>
+__device__ inline V8<double> jac_state(const V8<double>& wp, const V8<double>& wm) {
+    if constexpr (SPLIT) {
+        const V8<float> a = ROE ? roe_avg<float, MASK>(cast8<float, double, MASK>(wp), cast8<float, double, MASK>(wm))
+                                : arith_avg<float, MASK>(cast8<float, double, MASK>(wp), cast8<float, double, MASK>(wm));
+        return cast8<double, float, MASK>(a);
+    } else {
+        return ROE ? roe_avg<double, MASK>(wp, wm) : arith_avg<double, MASK>(wp, wm);
+    }
+}
+
 template <class R, int MASK, int SOLVER, int AX>
-__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma, int mom, bool physical) {
+__device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma, int mom, bool physical,
+                               bool split = false) {
     if constexpr (SOLVER == LLF)
         return flux_llf<R, MASK>(s, lam);
     else if constexpr (SOLVER == HLLC)
-        return flux_hllc<R, MASK>(s, sax, gamma, physical ? mom : 1, physical);
+        return split ? flux_hllc_split<MASK>(s, sax, gamma, physical ? mom : 1, physical)
+                     : flux_hllc<R, MASK>(s, sax, gamma, physical ? mom : 1, physical);
     else
         return sax == 0 ? flux_hlld<R, MASK, 0>(s, Bn, gamma) : flux_hlld<R, MASK, 1>(s, Bn, gamma);
 }
@@ -261,8 +295,9 @@ __device__ inline void block_max_atomic(double v, unsigned long long* dst, doubl
 // 1D: one fused kernel per stage: load -> cons->prim -> reconstruct/limit ->
 // spectral radius -> Riemann -> flux difference -> L = -dF/dx.
 // ============================================================================
-template <class R, int MASK, int RECON, int SOLVER, int TS>
+template <class R, int MASK, int RECON, int SOLVER, int TS, bool SPLIT = false>
 __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
+    static_assert(!SPLIT || sizeof(R) == 8, "split path keeps FP64 values");
     using V = VS<MASK>;
     constexpr int NV = V::NV;
     constexpr bool HO = (RECON == PPM || RECON >= WENO3);
@@ -281,12 +316,12 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     __shared__ R fl[NV][NF];
     __shared__ double red[4];
 
-    const int ntiles_ = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
-    const Tile tile = a.tiles[ti_];
-    const ProbDesc pd = a.pd[tile.prob];
+    const Tile tile = ld_const(a.tiles + ti_);
+    const ProbDesc pd = ld_const(a.pd + tile.prob);
     ProbState* ps = a.ps + tile.prob;
-    if (!ps->active) continue;  // uniform for the workgroup
+    if (!ld_const(&ps->active)) continue;
     const int N = pd.n0, bc = pd.bc, s0 = tile.s0;
     const double gamma = pd.gamma;
     const R gR = R(gamma);
@@ -306,7 +341,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) qd[V::slot(c)][i] = q.c[c];
         if constexpr (HO) {
-            const V8<double> w = cons_to_prim<double, MASK>(q, gamma);
+            const V8<double> w = c2p<MASK, SPLIT>(q, gamma);
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if (V::live(c)) cq[V::slot(c)][i] = w.c[c];
@@ -333,13 +368,14 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
                 return v;
             };
             const V8<double> z8 = zero8<double, MASK>();
-            w = ho_cell_prim<MASK, 1>(Q(iq), Q(iq - 1), Q(iq + 1), z8, z8, CQ(iq), CQ(iq - 1), CQ(iq + 1), z8, z8, gamma);
+            w = ho_cell_prim<MASK, 1, SPLIT>(Q(iq), Q(iq - 1), Q(iq + 1), z8, z8, CQ(iq), CQ(iq - 1), CQ(iq + 1), z8,
+                                             z8, gamma);
         } else {
             V8<double> q = zero8<double, MASK>();
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if (V::live(c)) q.c[c] = qd[V::slot(c)][iq];
-            w = cons_to_prim<double, MASK>(q, gamma);
+            w = c2p<MASK, SPLIT>(q, gamma);
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -404,9 +440,12 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         } else {
             const int je = bc == EDGE ? clampi(j, 1, N) : j;
             const V8<R> wp = WL(je), wm = WR(je - 1);
-            st = (RECON == PLM) ? arith_avg<R, MASK>(wp, wm) : roe_avg<R, MASK>(wp, wm);
+            if constexpr (SPLIT) st = jac_state<MASK, true, RECON != PLM>(wp, wm);
+            else st = (RECON == PLM) ? arith_avg<R, MASK>(wp, wm) : roe_avg<R, MASK>(wp, wm);
         }
-        const R r = spectral_radius<R, MASK, 0>(st, gR);
+        R r;
+        if constexpr (SPLIT) r = radius<MASK, 0, true>(st, gamma);
+        else r = spectral_radius<R, MASK, 0>(st, gR);
         rad[i] = r;
         if (j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= N) {
             const double rd = double(r);
@@ -435,16 +474,24 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             f.qp = Qc(k);
             f.wm = W(e(k - 1));
             f.wp = W(e(k));
+        } else if constexpr (SPLIT) {
+            f.qm = prim_to_cons_split<MASK>(f.wm, gamma);
+            f.qp = prim_to_cons_split<MASK>(f.wp, gamma);
         } else {
             f.qm = prim_to_cons<R, MASK>(f.wm, gR);
             f.qp = prim_to_cons<R, MASK>(f.wp, gR);
         }
-        f.fm = phys_flux<R, MASK, 0>(f.wm, gR);
-        f.fp = phys_flux<R, MASK, 0>(f.wp, gR);
+        if constexpr (SPLIT) {
+            f.fm = phys_flux_split<MASK, 0>(f.wm, gamma);
+            f.fp = phys_flux_split<MASK, 0>(f.wp, gamma);
+        } else {
+            f.fm = phys_flux<R, MASK, 0>(f.wm, gR);
+            f.fp = phys_flux<R, MASK, 0>(f.wp, gR);
+        }
         const R lam = nmax(rad[i], rad[i + 1]);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = W(e(k)).c[5];
-        const V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR, 1, a.compat != 0);
+        const V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR, 1, a.compat != 0, SPLIT);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) fl[V::slot(c)][i] = F.c[c];
@@ -475,7 +522,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
 // -> limited interface states wL/wR of the tile's own cells, written in the
 // sweep-frame layout [t][s] (s contiguous) of type R.
 // ============================================================================
-template <class R, int MASK, int RECON, int AX, int TS, int TT>
+template <class R, int MASK, int RECON, int AX, int TS, int TT, bool SPLIT = false>
 __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
     using V = VS<MASK>;
     constexpr int NV = V::NV;
@@ -489,12 +536,12 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
     __shared__ double cq[HO ? NV : 1][HO ? NQT : 1][HO ? NQS : 1];
     __shared__ R ws[NV][TT][NWS];
 
-    const int ntiles_ = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
-    const Tile tile = a.tiles[ti_];
-    const ProbDesc pd = a.pd[tile.prob];
+    const Tile tile = ld_const(a.tiles + ti_);
+    const ProbDesc pd = ld_const(a.pd + tile.prob);
     ProbState* ps = a.ps + tile.prob;
-    if (!ps->active) continue;  // uniform for the workgroup
+    if (!ld_const(&ps->active)) continue;
     const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
     const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
     const double gamma = pd.gamma;
@@ -519,7 +566,7 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) qd[V::slot(c)][it][is] = q.c[c];
         if constexpr (HO) {
-            const V8<double> w = cons_to_prim<double, MASK>(q, gamma);
+            const V8<double> w = c2p<MASK, SPLIT>(q, gamma);
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if (V::live(c)) cq[V::slot(c)][it][is] = w.c[c];
@@ -548,11 +595,11 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
                     if (V::live(c)) v.c[c] = cq[V::slot(c)][y][x];
                 return v;
             };
-            w = ho_cell_prim<MASK, 2>(Q(js, jt), Q(js - 1, jt), Q(js + 1, jt), Q(js, jt - 1), Q(js, jt + 1),
-                                      CQ(js, jt), CQ(js - 1, jt), CQ(js + 1, jt), CQ(js, jt - 1), CQ(js, jt + 1),
-                                      gamma);
+            w = ho_cell_prim<MASK, 2, SPLIT>(Q(js, jt), Q(js - 1, jt), Q(js + 1, jt), Q(js, jt - 1), Q(js, jt + 1),
+                                             CQ(js, jt), CQ(js - 1, jt), CQ(js + 1, jt), CQ(js, jt - 1), CQ(js, jt + 1),
+                                             gamma);
         } else {
-            w = cons_to_prim<double, MASK>(Q(js, jt), gamma);
+            w = c2p<MASK, SPLIT>(Q(js, jt), gamma);
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -579,7 +626,7 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
                     if (VS<MASK>::live(c)) v.c[c] = row[VS<MASK>::slot(c)][it][j];
                 return v;
             }
-        } W{ws, it, bws, ns, bc};
+            } W{ws, it, bws, ns, bc};
         V8<R> L, Rr;
         recon_cell<R, MASK, RECON, AX>(W, s, a.limiter, a.flattener, L, Rr);
         const long long o = fo + (long long)t * ns + s;
@@ -610,7 +657,7 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
             // state, the solver axis being the sweep's enumeration index
             // (solvers.py:160,168,183)
             if (a.bn) {
-                const int parity = a.parity_override >= 0 ? a.parity_override : ps->parity;
+                const int parity = a.parity_override >= 0 ? a.parity_override : ld_const(&ps->parity);
                 const int sax = a.compat ? AX : (parity == 0 ? AX : 1 - AX);
                 const V8<R> z = W(s);
                 reinterpret_cast<R*>(a.bn)[o] = sax == 0 ? z.c[5] : z.c[6];
@@ -628,7 +675,7 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
 // CONV: 0 = PCM (q from the cell state), 1 = point (PLM), 2 = 4th-order face
 // conversion (PPM/WENO, fv.py:105-122 with 'face').
 // ============================================================================
-template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT>
+template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT, bool SPLIT = false>
 __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
     using V = VS<MASK>;
     constexpr int NV = V::NV;
@@ -648,17 +695,17 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
     __shared__ R F_s[NV][TT][NKF];
     __shared__ double red[4];
 
-    const int ntiles_ = a.ntiles_dev ? *a.ntiles_dev : a.ntiles;
+    const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
-    const Tile tile = a.tiles[ti_];
-    const ProbDesc pd = a.pd[tile.prob];
+    const Tile tile = ld_const(a.tiles + ti_);
+    const ProbDesc pd = ld_const(a.pd + tile.prob);
     ProbState* ps = a.ps + tile.prob;
-    if (!ps->active) continue;  // uniform for the workgroup
+    if (!ld_const(&ps->active)) continue;
     const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
     const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
     const R gR = R(pd.gamma);
     const long long off = pd.off, P = a.plane;
-    const int parity = a.parity_override >= 0 ? a.parity_override : ps->parity;
+    const int parity = a.parity_override >= 0 ? a.parity_override : ld_const(&ps->parity);
     // solver axis: the enumeration index of the sweep (solvers.py:34-36), or
     // the physical axis in the physical-compat mode
     const int sax = a.compat ? AX : (parity == 0 ? AX : 1 - AX);
@@ -712,9 +759,12 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             const int je = bc == EDGE ? clampi(j, 1, ns) : j;
             const int jk = je - s0;  // in [0, NK) or the clamped edge interface
             const V8<R> wp = WP(it, jk), wm = WM(it, jk);
-            st = (CONV == 1) ? arith_avg<R, MASK>(wp, wm) : roe_avg<R, MASK>(wp, wm);
+            if constexpr (SPLIT) st = jac_state<MASK, true, CONV != 1>(wp, wm);
+            else st = (CONV == 1) ? arith_avg<R, MASK>(wp, wm) : roe_avg<R, MASK>(wp, wm);
         }
-        const R r = spectral_radius<R, MASK, AX>(st, gR);
+        R r;
+        if constexpr (SPLIT) r = radius<MASK, AX, true>(st, pd.gamma);
+        else r = spectral_radius<R, MASK, AX>(st, gR);
         rad_s[ia][ik] = r;
         if (t >= t0 && t < t0 + TT && t < nt && j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= ns) {
             const double rd = double(r);
@@ -747,19 +797,19 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             f.qm = cast8<R, double, MASK>(qm);
             f.qp = cast8<R, double, MASK>(qp);
         } else if constexpr (CONV == 1) {
-            f.qm = prim_to_cons<R, MASK>(f.wm, gR);
-            f.qp = prim_to_cons<R, MASK>(f.wp, gR);
+            f.qm = p2c<MASK, SPLIT>(f.wm, pd.gamma);
+            f.qp = p2c<MASK, SPLIT>(f.wp, pd.gamma);
         } else {
             // face conversion along t (fv.py:105-122, axes = [1] of the sweep frame):
             // q = P(w - K24 lap_t w) + K24 lap_t P(w)
             auto face = [&](const V8<R>& wc, const V8<R>& wlo, const V8<R>& whi) {
-                const V8<R> pc = prim_to_cons<R, MASK>(wc, gR), plo = prim_to_cons<R, MASK>(wlo, gR),
-                            phi = prim_to_cons<R, MASK>(whi, gR);
+                const V8<R> pc = p2c<MASK, SPLIT>(wc, pd.gamma), plo = p2c<MASK, SPLIT>(wlo, pd.gamma),
+                            phi = p2c<MASK, SPLIT>(whi, pd.gamma);
                 V8<R> ww = wc;
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
                     if (V::live(c)) ww.c[c] = wc.c[c] - R(K24) * ((whi.c[c] - wc.c[c]) - (wc.c[c] - wlo.c[c]));
-                V8<R> q = prim_to_cons<R, MASK>(ww, gR);
+                V8<R> q = p2c<MASK, SPLIT>(ww, pd.gamma);
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
                     if (V::live(c)) q.c[c] = q.c[c] + R(K24) * ((phi.c[c] - pc.c[c]) - (pc.c[c] - plo.c[c]));
@@ -769,8 +819,8 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             f.qm = face(f.wm, WM(itm, ik), WM(itp, ik));
             f.qp = face(f.wp, WP(itm, ik), WP(itp, ik));
         }
-        f.fm = phys_flux<R, MASK, AX>(f.wm, gR);
-        f.fp = phys_flux<R, MASK, AX>(f.wp, gR);
+        f.fm = pflux<MASK, AX, SPLIT>(f.wm, pd.gamma);
+        f.fp = pflux<MASK, AX, SPLIT>(f.wp, pd.gamma);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) {
             const int gt = bcmap(te, nt, bc);
@@ -778,7 +828,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             bn_s[ia][ik] = Bn;
         }
         const R lam = nmax(rad_s[ia][ik], rad_s[ia][ik + 1]);
-        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0);
+        const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0, SPLIT);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) {
@@ -823,7 +873,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
         const R lam = nmax(rad_s[ia0][ik], rad_s[ia0][ik + 1]);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = bn_s[ia0][ik];
-        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0);
+        const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0, SPLIT);
         const int k = s0 + ik;
 #pragma unroll
         for (int c = 0; c < 8; ++c)

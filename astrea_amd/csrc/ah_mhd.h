@@ -37,14 +37,14 @@ __global__ __launch_bounds__(256) void trecon_kernel(MhdArgs a) {
     constexpr int NV = V::NV;
     constexpr int H = 4, TW = TT + 2 * H;
     __shared__ R win[NV][TW][TS];
-    const int ntiles = *a.nttiles_dev[AX];
+    const int ntiles = wave_uniform(*a.nttiles_dev[AX]);
     const R* src = reinterpret_cast<const R*>(a.src[AX]);
     R* oD = reinterpret_cast<R*>(a.wD[AX]);
     R* oU = reinterpret_cast<R*>(a.wU[AX]);
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tile = a.ttiles[AX][ti];
-        const ProbDesc pd = a.pd[tile.prob];
-        if (!a.ps[tile.prob].active) continue;
+        const Tile tile = ld_const(a.ttiles[AX] + ti);
+        const ProbDesc pd = ld_const(a.pd + tile.prob);
+        if (!ld_const(&a.ps[tile.prob].active)) continue;
         const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
         const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
         const long long off = pd.off, P = a.plane;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void trecon_kernel(MhdArgs a) {
                         if (VS<MASK>::live(c)) r.c[c] = w[VS<MASK>::slot(c)][j][is];
                     return r;
                 }
-            } W{win, is, t0 - H, nt, bc};
+                    } W{win, is, t0 - H, nt, bc};
             V8<R> D, Uu;
             recon_cell<R, MASK, PPM, AX>(W, t, MINMOD, FLAT_OFF, D, Uu);
             const long long o = off + (long long)t * ns + s;
@@ -122,17 +122,17 @@ __device__ inline void corner_speeds(const V8<R>& plus, const V8<R>& minus, int 
 template <class R, int MASK>
 __global__ __launch_bounds__(256) void corner_kernel(MhdArgs a) {
     using V = VS<MASK>;
-    const int ntiles = *a.ntiles_dev;
+    const int ntiles = wave_uniform(*a.ntiles_dev);
     const R* Dx = reinterpret_cast<const R*>(a.wD[0]);
     const R* Ux = reinterpret_cast<const R*>(a.wU[0]);
     const R* Dy = reinterpret_cast<const R*>(a.wD[1]);
     const R* Uy = reinterpret_cast<const R*>(a.wU[1]);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = a.tiles[ti];
+        const Tile tl = ld_const(a.tiles + ti);
         const ProbState& st = a.ps[tl.prob];
-        if (!st.active) continue;
-        const ProbDesc pd = a.pd[tl.prob];
+        if (!ld_const(&st.active)) continue;
+        const ProbDesc pd = ld_const(a.pd + tl.prob);
         const int n0 = pd.n0, n1 = pd.n1;
         const R gamma = R(pd.gamma);
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
@@ -179,12 +179,12 @@ __global__ __launch_bounds__(256) void corner_kernel(MhdArgs a) {
 template <int MASK>
 __global__ __launch_bounds__(256) void magL_kernel(MhdArgs a) {
     using V = VS<MASK>;
-    const int ntiles = *a.ntiles_dev;
+    const int ntiles = wave_uniform(*a.ntiles_dev);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = a.tiles[ti];
-        if (!a.ps[tl.prob].active) continue;
-        const ProbDesc pd = a.pd[tl.prob];
+        const Tile tl = ld_const(a.tiles + ti);
+        if (!ld_const(&a.ps[tl.prob].active)) continue;
+        const ProbDesc pd = ld_const(a.pd + tl.prob);
         const int n0 = pd.n0, n1 = pd.n1;
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
             const int cell = tl.s0 + c;
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void magL_kernel(MhdArgs a) {
 template <class R, int MASK>
 __global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
     using V = VS<MASK>;
-    const int ntiles = *a.ntiles_dev;
+    const int ntiles = wave_uniform(*a.ntiles_dev);
     const long long P = a.plane;
     const R* Fx = reinterpret_cast<const R*>(a.src[0]);
     const R* Fy = reinterpret_cast<const R*>(a.src[1]);
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
 template <int MASK>
 __global__ __launch_bounds__(256) void refine_kernel(MhdArgs a) {
     using V = VS<MASK>;
-    const int ntiles = *a.ntiles_dev;
+    const int ntiles = wave_uniform(*a.ntiles_dev);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const Tile tl = a.tiles[ti];
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void refine_kernel(MhdArgs a) {
 template <int MASK>
 __global__ __launch_bounds__(256) void copyB_kernel(MhdArgs a, double* dst) {
     using V = VS<MASK>;
-    const int ntiles = *a.ntiles_dev;
+    const int ntiles = wave_uniform(*a.ntiles_dev);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const Tile tl = a.tiles[ti];

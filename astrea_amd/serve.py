@@ -147,9 +147,9 @@ class Server:
         self.device = device
         self.keep_results = keep_results
         self.groups = {}
-        # state + stage registers + interface states per cell (2D hydro, SSPRK(3,3))
-        rb = 8 if precision == "fp64" else 4
-        self.bytes_per_cell = bytes_per_cell or (4 * 8 * 5 + 2 * 4 * rb + rb)
+        # state + stage registers + interface states per cell (2D hydro, SSPRK(3,3));
+        # both precision modes keep FP64 values in HBM
+        self.bytes_per_cell = bytes_per_cell or (4 * 8 * 5 + 2 * 4 * 8 + 8)
 
     def cost(self, r):
         return r.n * r.n * self.bytes_per_cell

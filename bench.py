@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024, help="grid cells per axis")
     ap.add_argument("--paths", default="fp64,split")
+    ap.add_argument("--blocks", type=int, default=0, help="persistent grid cap per launch (0: plan default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -191,6 +192,8 @@ def main():
     for path in args.paths.split(","):
         plan = Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], flattener="astrea", precision=path, varmask=0x17,
                     compat="physical")
+        if args.blocks:
+            plan.set_blocks(args.blocks)
         plan.load([q0])
         plan.reset()
         plan.step(args.warmup)
