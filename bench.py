@@ -52,6 +52,10 @@ def parse():
     return ap.parse_args()
 
 
+def _progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(n, steps):
     """The oracle (numpy restatement of the reference, with the reference's
     LAPACK eigenvalue solve per interface) on a bounded sample of the same
@@ -104,18 +108,23 @@ def bench_ensemble(args):
     res = {}
     for prec in ("fp64", "split"):
         res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, chunk=16, blocks=blocks)
+        _progress(f"[ensemble] batched {prec}: {res[prec]['zu_per_s']:.3e} ZU/s")
     literal = W.run_ensemble_batched(grids, probs, precision="fp64", chunk=16, blocks=blocks, compat="astrea")
     ref = res["fp64"]
-    errs = []
+    errs, mism = [], 0
     for i in range(len(probs)):
-        if ref["status"][i]["failed"] or res["split"]["status"][i]["failed"]:
+        if bool(ref["status"][i]["failed"]) != bool(res["split"]["status"][i]["failed"]):
+            mism += 1
+            continue
+        if ref["status"][i]["failed"]:
             continue
         o, n = ref["offsets"][i], probs[i]["n0"]
         errs.append(W.rel_l1(res["split"]["state"][:, o:o + n], ref["state"][:, o:o + n]))
     rng = np.random.default_rng(SEED)
     sample = sorted(rng.choice(len(probs), min(args.plain_sample, len(probs)), replace=False).tolist())
     plain = W.run_ensemble_plain(grids, probs, sample=sample)
-    split_ok = bool(errs) and max(errs) <= 1e-5
+    _progress(f"[ensemble] plain fp64 sample: {plain['zu_per_s']:.3e} ZU/s")
+    split_ok = bool(errs) and max(errs) <= 1e-5 and mism == 0
     best = "split" if split_ok and res["split"]["zu_per_s"] > ref["zu_per_s"] else "fp64"
     value = res[best]["zu_per_s"]
     return {
@@ -133,37 +142,73 @@ def bench_ensemble(args):
                                         "failed": sum(s["failed"] for s in literal["status"])},
         "speedup_vs_plain": value / plain["zu_per_s"],
         "split_l1_vs_fp64": {"max": max(errs) if errs else None, "median": float(np.median(errs)) if errs else None,
-                             "p90": float(np.percentile(errs, 90)) if errs else None},
+                             "p90": float(np.percentile(errs, 90)) if errs else None, "compared": len(errs),
+                             "failure_mismatch": mism},
         "autotune_blocks": {"picked": blocks, "seconds": {str(k): v for k, v in tune.items()}},
     }
 
 
 def bench_serve(args):
-    """Poisson stream of 2D four-quadrant requests served
-    with continuous batching, against the plain FIFO FP64 loop on the same
-    arrival trace."""
+    """stream of 2D four-quadrant requests served with
+    continuous batching (FP64 and split), against the plain FIFO FP64 loop on
+    the same requests.  Throughput comes from the saturated stream (every
+    request queued at t=0, so zone-updates/s measures the server, not the
+    arrival rate); latency p50/p99 from the real-time Poisson trace.  Each
+    served result is compared with the plain loop's result for the same
+    request (rel L1, max over variables)."""
+    import numpy as np
     from astrea_amd import serve
+    from astrea_amd.workloads import rel_l1
     nreq = max(1, int(args.duration * args.rate))
-    out = {}
-    for mode in ("served", "plain"):
+
+    def requests(realtime):
         reqs = serve.make_requests(nreq, SEED, args.rate)
-        if mode == "served":
-            r = serve.Server(precision="fp64").run(reqs)
-        else:
-            r = serve.run_fifo(reqs)
-        r.pop("done")
-        out[mode] = r
-    s = out["served"]
+        if not realtime:
+            for r in reqs:
+                r.arrival = 0.0
+        return reqs
+
+    modes = {}
+    for mode, realtime in (("saturated", False), ("poisson", True)):
+        plain = serve.run_fifo(requests(realtime), realtime=realtime, keep_results=True)
+        ref = {r.rid: r for r in plain.pop("done")}
+        out = {"plain_fp64": plain}
+        _progress(f"[serve] {mode} plain fp64: {plain['zu_per_s']:.3e} ZU/s, p50 {plain['p50']:.2f} s")
+        for prec in ("fp64", "split"):
+            r = serve.Server(precision=prec, keep_results=True).run(requests(realtime), realtime=realtime)
+            done = r.pop("done")
+            errs, mism = [], 0
+            for x in done:
+                y = ref[x.rid]
+                if bool(x.status["failed"]) != bool(y.status["failed"]):
+                    mism += 1
+                elif not y.status["failed"]:
+                    errs.append(rel_l1(x.result, y.result))
+            r["l1_vs_plain"] = {"max": max(errs) if errs else None,
+                                "median": float(np.median(errs)) if errs else None, "compared": len(errs),
+                                "failure_mismatch": mism}
+            out[f"served_{prec}"] = r
+            _progress(f"[serve] {mode} served {prec}: {r['zu_per_s']:.3e} ZU/s, p50 {r['p50']:.2f} s, "
+                      f"L1 max {r['l1_vs_plain']['max']}")
+        modes[mode] = out
+    sat = modes["saturated"]
+    ok = [p for p in ("served_split", "served_fp64")
+          if sat[p]["l1_vs_plain"]["max"] is not None and sat[p]["l1_vs_plain"]["max"] <= 1e-5
+          and sat[p]["l1_vs_plain"]["failure_mismatch"] == 0]
+    best = max(ok, key=lambda p: sat[p]["zu_per_s"]) if ok else "served_fp64"
+    value = sat[best]["zu_per_s"]
+    poi = modes["poisson"]
     return {
-        "metric": METRIC, "value": s["zu_per_s"], "unit": "zone-updates/s", "n_gpus": 1, "steps": None,
+        "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": None,
         "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{nreq} four-quadrant 2D requests, Poisson {args.rate}/s for {args.duration:.0f} s, "
-                               "n in {128,256,512,1024}^2, PLM or PPM, HLLC (physical axis), SSPRK(3,3), CFL 0.4, "
-                               "t_end 0.25, edge", "seed": SEED},
-        "served": out["served"], "plain_fifo": out["plain"],
-        "p50_s": s["p50"], "p99_s": s["p99"], "plain_p50_s": out["plain"]["p50"], "plain_p99_s": out["plain"]["p99"],
-        "speedup_vs_plain": s["zu_per_s"] / out["plain"]["zu_per_s"],
+        "dtype": "f64" if best == "served_fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{nreq} four-quadrant 2D requests (saturated queue for throughput; Poisson "
+                               f"{args.rate}/s for latency), n in {{128,256,512,1024}}^2, PLM or PPM, HLLC (physical "
+                               "axis), SSPRK(3,3), CFL 0.4, t_end 0.25, edge", "path": best, "seed": SEED},
+        "saturated": sat, "poisson": poi,
+        "speedup_vs_plain": value / sat["plain_fp64"]["zu_per_s"],
+        "p50_s": poi[best]["p50"], "p99_s": poi[best]["p99"],
+        "plain_p50_s": poi["plain_fp64"]["p50"], "plain_p99_s": poi["plain_fp64"]["p99"],
     }
 
 
