@@ -146,68 +146,59 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
             wR.c[c] = z.c[c] + g;
         }
     } else if constexpr (RECON == PPM) {
-        // window values at ce-4 .. ce+4 (W applies the boundary rule)
-        V8<R> w[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) w[k] = W(ce - 4 + k);
-        // face interpolant wF at ce-1 and ce: wF[i] = 7/12 (w_i + w_i+1) - 1/12 (w_i-1 + w_i+2)
-        // (ppm.py:38); the reference pads wF itself, so wF at position m is
-        // evaluated at W.e(m) -- W.at_e handles that.
-        auto wF_at = [&](int m) {
-            const int me = W.e(m);
-            const V8<R> a = W(me - 1), b = W(me), c1 = W(me + 1), d = W(me + 2);
-            V8<R> f = zero8<R, MASK>();
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                if (V::live(c)) f.c[c] = R(K7_12) * (b.c[c] + c1.c[c]) - R(K1_12) * (a.c[c] + d.c[c]);
-            if (flattener != FLAT_OFF) {
-                // eta(me) = min(chi(me), chi(me -/+ 1)) by sign of the pressure jump (ppm.py:127-131)
-                auto chi_at = [&](int mm) {
-                    const int e2 = W.e(mm);
-                    const V8<R> p2 = W(e2 - 2), p1 = W(e2 - 1), q1 = W(e2 + 1), q2 = W(e2 + 2);
-                    return flat_chi<R>(flattener, p2.c[4], p1.c[4], q1.c[4], q2.c[4], p1.c[AX + 1], q1.c[AX + 1]);
-                };
+        // The flattener's eta at the two faces depends on pressure and the
+        // normal velocity only; it is computed once, then every component runs
+        // its own chain (interpolant, curvatures, third differences, limiter),
+        // which keeps one component's window in registers at a time.  Every
+        // padded intermediate at position m is evaluated at W.e(m).
+        R etaL = R(1), etaR = R(1);
+        if (flattener != FLAT_OFF) {
+            // chi~ (ppm.py:111-124) at me, from p(me-2..me+2) and u(me-1), u(me+1)
+            auto chi_at = [&](int mm) {
+                const int e2 = W.e(mm);
+                const V8<R> p2 = W(e2 - 2), p1 = W(e2 - 1), q1 = W(e2 + 1), q2 = W(e2 + 2);
+                return flat_chi<R>(flattener, p2.c[4], p1.c[4], q1.c[4], q2.c[4], p1.c[AX + 1], q1.c[AX + 1]);
+            };
+            // eta(me) = min(chi(me), chi(me -/+ 1)) by sign of the pressure jump (ppm.py:127-131)
+            auto eta_at = [&](int m) {
+                const int me = W.e(m);
                 const R chi0 = chi_at(me);
-                const R sg = nsign(c1.c[4] - a.c[4]);
+                const R sg = nsign(W(me + 1).c[4] - W(me - 1).c[4]);
                 R eta = chi0;
                 if (sg < R(0)) eta = nmin(chi0, chi_at(me + 1));
                 if (sg > R(0)) eta = nmin(chi0, chi_at(me - 1));
-#pragma unroll
-                for (int c = 0; c < 8; ++c)
-                    if (V::live(c)) f.c[c] = f.c[c] * eta + b.c[c] * (R(1) - eta);
-            }
-            return f;
-        };
-        auto d2_at = [&](int m) {
-            const int me = W.e(m);
-            const V8<R> a = W(me - 1), b = W(me), c1 = W(me + 1);
-            V8<R> d = zero8<R, MASK>();
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                if (V::live(c)) d.c[c] = a.c[c] - R(2) * b.c[c] + c1.c[c];
-            return d;
-        };
-        const V8<R> fL = wF_at(ce - 1), fR = wF_at(ce);
-        // curvatures d2w_C padded by one (limiters.py:93,103): at e(ce-1), ce, e(ce+1)
-        const V8<R> d2m = d2_at(ce - 1), d2c = d2_at(ce), d2p = d2_at(ce + 1);
-        // third differences d3w[m] = d2w_C[e(m+1)] - d2w_C[m] (limiters.py:96), padded
-        // by two (126-127): needed at e(ce-2), e(ce-1), ce, e(ce+2)
-        auto d3_at = [&](int m) {
-            const int me = W.e(m);
-            const V8<R> hi = d2_at(me + 1), lo = d2_at(me);
-            V8<R> d = zero8<R, MASK>();
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                if (V::live(c)) d.c[c] = hi.c[c] - lo.c[c];
-            return d;
-        };
-        const V8<R> d3m2 = d3_at(ce - 2), d3m1 = d3_at(ce - 1), d30 = d3_at(ce), d3p2 = d3_at(ce + 2);
-        const V8<R>& z = w[4];
+                return eta;
+            };
+            etaL = eta_at(ce - 1);
+            etaR = eta_at(ce);
+        }
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             if (!V::live(c)) continue;
-            mc_limit<R>(z.c[c], w[3].c[c], w[5].c[c], w[2].c[c], w[6].c[c], fL.c[c], fR.c[c], d2m.c[c], d2c.c[c],
-                        d2p.c[c], d3m2.c[c], d3m1.c[c], d30.c[c], d3p2.c[c], wL.c[c], wR.c[c]);
+            auto w = [&](int m) { return W(m).c[c]; };
+            // face interpolant at me (ppm.py:38), blended toward w(me) by eta (ppm.py:65-67)
+            auto wF_at = [&](int m, R eta) {
+                const int me = W.e(m);
+                const R b = w(me);
+                R f = R(K7_12) * (b + w(me + 1)) - R(K1_12) * (w(me - 1) + w(me + 2));
+                if (flattener != FLAT_OFF) f = f * eta + b * (R(1) - eta);
+                return f;
+            };
+            auto d2_at = [&](int m) {
+                const int me = W.e(m);
+                return w(me - 1) - R(2) * w(me) + w(me + 1);
+            };
+            // d3w[m] = d2w_C[e(m+1)] - d2w_C[m] (limiters.py:96), padded by two (126-127)
+            auto d3_at = [&](int m) {
+                const int me = W.e(m);
+                return d2_at(me + 1) - d2_at(me);
+            };
+            const R fL = wF_at(ce - 1, etaL), fR = wF_at(ce, etaR);
+            // curvatures d2w_C padded by one (limiters.py:93,103): at e(ce-1), ce, e(ce+1)
+            const R d2m = d2_at(ce - 1), d2c = d2_at(ce), d2p = d2_at(ce + 1);
+            const R d3m2 = d3_at(ce - 2), d3m1 = d3_at(ce - 1), d30 = d3_at(ce), d3p2 = d3_at(ce + 2);
+            mc_limit<R>(w(ce), w(ce - 1), w(ce + 1), w(ce - 2), w(ce + 2), fL, fR, d2m, d2c, d2p, d3m2, d3m1, d30,
+                        d3p2, wL.c[c], wR.c[c]);
         }
     } else {
         constexpr int H = RECON == WENO3 ? 1 : (RECON == WENO5 ? 2 : 3);
