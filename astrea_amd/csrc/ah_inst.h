@@ -20,7 +20,11 @@ inline void set2d(KernelSet* ks) {
     constexpr int CONV = RECON == PCM ? 0 : (RECON == PLM ? 1 : 2);
     constexpr int NVB = VS<MASK>::NV * (int)sizeof(R);
     constexpr int RTS0 = 32, RTT0 = 8, RTS1 = 64, RTT1 = 4;
-    constexpr int FTS = NVB > 32 ? 16 : 32, FTT = 4;
+    // flux tile: (FTT + 2) x (FTS + 1) face solves and (FTT + 2) x (FTS + 2)
+    // radii per tile each fit one pass of 256 threads, at two workgroups per CU
+    // of LDS (hydro 16 x 12: 65 KB; MHD 16 x 8: 71 KB); measured on the 1024^2
+    // shear layer against 32 x 4, 24 x 8 and 40 x 4
+    constexpr int FTS = 16, FTT = NVB > 32 ? 8 : 12;
     ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
         launch_sweep(recon2d_kernel<R, MASK, RECON, 0, RTS0, RTT0, SPLIT>, a, nb, s);
     };
