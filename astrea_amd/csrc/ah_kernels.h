@@ -12,6 +12,8 @@
 #pragma once
 #include "ah_device.h"
 
+#include <type_traits>
+
 namespace ah {
 
 struct ProbDesc {
@@ -286,6 +288,27 @@ __device__ inline void block_max_atomic(double v, unsigned long long* dst, doubl
 // 1D: one fused kernel per stage: load -> cons->prim -> reconstruct/limit ->
 // spectral radius -> Riemann -> flux difference -> L = -dF/dx.
 // ============================================================================
+// Window accessor of the 1D sweep: W(m) is the primitive state at e(m);
+// CLAMP = false when the tile's stencil never leaves the domain or the
+// boundary is periodic (e is then the identity).
+template <class R, int MASK, int NW, bool CLAMP>
+struct LineAcc {
+    R (*ws)[NW];
+    int bw, N;
+    __device__ int e(int p) const {
+        if constexpr (CLAMP) return clampi(p, 0, N - 1);
+        else return p;
+    }
+    __device__ V8<R> operator()(int m) const {
+        const int j = e(m) - bw;
+        V8<R> v = zero8<R, MASK>();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (VS<MASK>::live(c)) v.c[c] = ws[VS<MASK>::slot(c)][j];
+        return v;
+    }
+};
+
 template <class R, int MASK, int RECON, int SOLVER, int TS, bool SPLIT = false>
 __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     static_assert(!SPLIT || sizeof(R) == 8, "split path keeps FP64 values");
@@ -386,18 +409,27 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             return v;
         }
     } W{ws, bw, N, bc};
-    // C: reconstruction at cells [s0-1, s0+TS+2)
+    // C: reconstruction at cells [s0-1, s0+TS+2); their +-4 stencils cover
+    // the window [s0-5, s0+TS+6), so tiles with that window inside the domain
+    // (and every periodic tile) skip the boundary map
     if constexpr (RECON != PCM) {
-        for (int i = threadIdx.x; i < NC; i += blockDim.x) {
-            V8<R> L, Rr;
-            recon_cell<R, MASK, RECON, 0>(W, e(bcl + i), a.limiter, a.flattener, L, Rr);
+        const bool clampC = bc == EDGE && !(s0 >= HW_L && s0 + TS + HW_R <= N);
+        auto recon_cells = [&](auto clamp_tag) {
+            constexpr bool CL = decltype(clamp_tag)::value;
+            const LineAcc<R, MASK, NW, CL> Wc{ws, bw, N};
+            for (int i = threadIdx.x; i < NC; i += blockDim.x) {
+                V8<R> L, Rr;
+                recon_cell<R, MASK, RECON, 0>(Wc, Wc.e(bcl + i), a.limiter, a.flattener, L, Rr);
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
-                if (V::live(c)) {
-                    wl[V::slot(c)][i] = L.c[c];
-                    wr[V::slot(c)][i] = Rr.c[c];
-                }
-        }
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) {
+                        wl[V::slot(c)][i] = L.c[c];
+                        wr[V::slot(c)][i] = Rr.c[c];
+                    }
+            }
+        };
+        if (clampC) recon_cells(std::true_type{});
+        else recon_cells(std::false_type{});
         __syncthreads();
     }
     auto WL = [&](int m) {  // wL at cell m (stored values already follow the rule)
@@ -508,6 +540,29 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     }
 }
 
+// Window accessor along the sweep for one line of a 2D tile: W(m) is the
+// primitive state at sweep position e(m).  CLAMP = false when the tile's
+// stencil (+-4 cells) never leaves the domain or the boundary is periodic
+// (the window then holds the wrapped cells): e is the identity and repeated
+// positions are recognised as such by the compiler.
+template <class R, int MASK, int TT, int NWS, bool CLAMP>
+struct RowAcc {
+    R (*row)[TT][NWS];
+    int it, bws, n;
+    __device__ int e(int p) const {
+        if constexpr (CLAMP) return clampi(p, 0, n - 1);
+        else return p;
+    }
+    __device__ V8<R> operator()(int m) const {
+        const int j = e(m) - bws;
+        V8<R> v = zero8<R, MASK>();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (VS<MASK>::live(c)) v.c[c] = row[VS<MASK>::slot(c)][it][j];
+        return v;
+    }
+};
+
 // ============================================================================
 // 2D reconstruction kernel: stage input -> wS (4th-order or point conversion)
 // -> limited interface states wL/wR of the tile's own cells, written in the
@@ -601,23 +656,16 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
     R* wLo = reinterpret_cast<R*>(a.wL);
     R* wRo = reinterpret_cast<R*>(a.wR);
     const long long fo = off;  // same packing as the state, sweep-frame order
+    // edge tiles whose +-4 stencil reaches past the domain evaluate the
+    // boundary rule; all other tiles (and every periodic tile) do not
+    const bool clampS = bc == EDGE && !(s0 >= HS && s0 + TS + HS <= ns);
+    auto cells = [&](auto clamp_tag) {
+    constexpr bool CL = decltype(clamp_tag)::value;
     for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
         const int it = i / TS, is = i % TS;
         const int s = s0 + is, t = t0 + it;
         if (s >= ns || t >= nt) continue;
-        struct WAcc {
-            R (*row)[TT][NWS];
-            int it, bws, n, bc;
-            __device__ int e(int p) const { return bc == EDGE ? clampi(p, 0, n - 1) : p; }
-            __device__ V8<R> operator()(int m) const {
-                const int j = e(m) - bws;
-                V8<R> v = zero8<R, MASK>();
-#pragma unroll
-                for (int c = 0; c < 8; ++c)
-                    if (VS<MASK>::live(c)) v.c[c] = row[VS<MASK>::slot(c)][it][j];
-                return v;
-            }
-            } W{ws, it, bws, ns, bc};
+        const RowAcc<R, MASK, TT, NWS, CL> W{ws, it, bws, ns};
         V8<R> L, Rr;
         recon_cell<R, MASK, RECON, AX>(W, s, a.limiter, a.flattener, L, Rr);
         const long long o = fo + (long long)t * ns + s;
@@ -655,6 +703,9 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
             }
         }
     }
+    };
+    if (clampS) cells(std::true_type{});
+    else cells(std::false_type{});
     __syncthreads();  // the next tile reuses the LDS windows
     }
 }
@@ -700,7 +751,6 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
     // solver axis: the enumeration index of the sweep (solvers.py:34-36), or
     // the physical axis in the physical-compat mode
     const int sax = a.compat ? AX : (parity == 0 ? AX : 1 - AX);
-    auto et = [&](int p) { return bc == EDGE ? clampi(p, 0, nt - 1) : p; };
     const int bt = t0 - 2, bta = t0 - 1;
     const R* wLg = reinterpret_cast<const R*>(a.wL);
     const R* wRg = reinterpret_cast<const R*>(a.wR);
@@ -734,6 +784,15 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             if (V::live(c)) w.c[c] = wp_s[V::slot(c)][it][ik];
         return w;
     };
+    // Tiles whose lines [t0-2, t0+TT+2) and Jacobian positions [s0, s0+TS+1]
+    // stay inside the domain (and every periodic tile) skip the boundary map.
+    const bool clampT = bc == EDGE && !(t0 >= 2 && t0 + TT + 2 <= nt && s0 >= 1 && s0 + TS + 1 <= ns);
+    auto stages = [&](auto clamp_tag) {
+    constexpr bool CL = decltype(clamp_tag)::value;
+    auto et = [&](int p) {
+        if constexpr (CL) return clampi(p, 0, nt - 1);
+        else return p;
+    };
     // B: spectral radius at padded Jacobian positions j in [s0, s0+TS+1],
     // lines [t0-1, t0+TT+1) (evaluated on the boundary-mapped line)
     double lmax = 0.0;
@@ -747,7 +806,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
             // PCM: Jacobian at cell j-1 = w_minus at interface j
             st = WM(it, ik);
         } else {
-            const int je = bc == EDGE ? clampi(j, 1, ns) : j;
+            const int je = CL ? clampi(j, 1, ns) : j;
             const int jk = je - s0;  // in [0, NK) or the clamped edge interface
             const V8<R> wp = WP(it, jk), wm = WM(it, jk);
             if constexpr (SPLIT) st = jac_state<MASK, true, CONV != 1>(wp, wm);
@@ -876,6 +935,9 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
                     a.flux_out[v * a.flux_plane + (long long)k * nt + t] = double(F);
             }
     }
+        };
+    if (clampT) stages(std::true_type{});
+    else stages(std::false_type{});
     __syncthreads();
     // E: flux difference; first sweep writes D, second writes L = -(D0 + D1)
     const R dxR = R(pd.dx);

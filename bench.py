@@ -42,9 +42,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--config", default="shear", choices=["shear", "ensemble", "serve"],
+    ap.add_argument("--config", default="shear", choices=["shear", "ensemble", "serve", "copy"],
                     help="shear: 2D shear layer (default); ensemble: 512 ragged 1D Riemann problems; "
-                         "serve: Poisson stream of 2D four-quadrant solves")
+                         "serve: Poisson stream of 2D four-quadrant solves; copy: device copy of --bytes "
+                         "(calibrates the HBM traffic counters)")
+    ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
     ap.add_argument("--nprob", type=int, default=512)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
@@ -212,12 +214,39 @@ def bench_serve(args):
     }
 
 
+def bench_copy(args):
+    """Device-to-device copy of a known byte count: reads and writes exactly
+    --bytes each.  Its FETCH_SIZE / WRITE_SIZE counters per dispatch give the
+    scale between those counters and HBM bytes."""
+    import torch
+    n = args.bytes // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    assert torch.equal(dst, src)
+    return {"metric": "device copy bandwidth", "value": 2 * n * 8 / (ms / 1e3) / 1e9, "unit": "GB/s",
+            "bytes_read": n * 8, "bytes_written": n * 8, "ms_per_copy": ms, "steps": args.steps,
+            "warmup": args.warmup}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a design "
               "note in DESIGN.md, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.config == "copy":
+        print(json.dumps(bench_copy(args)))
+        return
     if args.config != "shear":
         line = bench_ensemble(args) if args.config == "ensemble" else bench_serve(args)
         print(json.dumps(line))
