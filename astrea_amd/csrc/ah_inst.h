@@ -24,7 +24,7 @@ inline void set2d(KernelSet* ks) {
     // radii per tile each fit one pass of 256 threads, at two workgroups per CU
     // of LDS (hydro 16 x 12: 65 KB; MHD 16 x 8: 71 KB); measured on the 1024^2
     // shear layer against 32 x 4, 24 x 8 and 40 x 4
-    constexpr int FTS = 16, FTT = NVB > 32 ? 8 : 12;
+    constexpr int FTS = 16, FTT = NVB > 32 ? 8 : 12, FNT = 256;
     ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
         launch_sweep(recon2d_kernel<R, MASK, RECON, 0, RTS0, RTT0, SPLIT>, a, nb, s);
     };
@@ -32,10 +32,10 @@ inline void set2d(KernelSet* ks) {
         launch_sweep(recon2d_kernel<R, MASK, RECON, 1, RTS1, RTT1, SPLIT>, a, nb, s);
     };
     ks->flux[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 0, FTS, FTT, SPLIT>, a, nb, s);
+        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 0, FTS, FTT, SPLIT, FNT>, a, nb, s, FNT);
     };
     ks->flux[1] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 1, FTS, FTT, SPLIT>, a, nb, s);
+        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 1, FTS, FTT, SPLIT, FNT>, a, nb, s, FNT);
     };
     ks->rts[0] = RTS0;
     ks->rtt[0] = RTT0;

@@ -717,8 +717,8 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
 // CONV: 0 = PCM (q from the cell state), 1 = point (PLM), 2 = 4th-order face
 // conversion (PPM/WENO, fv.py:105-122 with 'face').
 // ============================================================================
-template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT, bool SPLIT = false>
-__global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
+template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT, bool SPLIT = false, int NTHR = 256>
+__global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     using V = VS<MASK>;
     constexpr int NV = V::NV;
     constexpr int NK = TS + 2;   // interfaces [s0, s0+TS+1]
@@ -735,7 +735,7 @@ __global__ __launch_bounds__(256) void flux2d_kernel(SweepArgs a) {
     __shared__ R rad_s[NTA][NK];
     __shared__ R bn_s[SOLVER == HLLD ? NTA : 1][SOLVER == HLLD ? NKF : 1];
     __shared__ R F_s[NV][TT][NKF];
-    __shared__ double red[4];
+    __shared__ double red[NTHR / 64];
 
     const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {

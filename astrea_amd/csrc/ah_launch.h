@@ -36,9 +36,9 @@ bool kernels_1d_fp32(int mask, int recon, int solver, KernelSet* ks);
 bool kernels_2d_fp32(int mask, int recon, int solver, KernelSet* ks);
 
 template <class K>
-inline void launch_sweep(K kernel, const SweepArgs& a, int nblocks, hipStream_t s) {
+inline void launch_sweep(K kernel, const SweepArgs& a, int nblocks, hipStream_t s, int nthreads = 256) {
     if (nblocks <= 0) return;
-    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(kernel, dim3(nblocks), dim3(nthreads), 0, s, a);
 }
 
 }  // namespace ah
