@@ -18,7 +18,7 @@ namespace ah {
 
 enum { EDGE = 0, WRAP = 1 };
 enum { PCM = 0, PLM = 1, PPM = 2, WENO3 = 3, WENO5 = 4, WENO7 = 5 };
-enum { LLF = 0, HLLC = 1, HLLD = 2 };
+enum { LLF = 0, HLLC = 1, HLLD = 2, DOTS = 3 };
 enum { MINMOD = 0, VANLEER = 1, OSPRE = 2, VANALBADA1 = 3, KOREN = 4, SUPERBEE = 5 };
 enum { FLAT_OFF = 0, FLAT_ASTREA = 1, FLAT_COLELLA90 = 2 };
 
@@ -691,6 +691,234 @@ __device__ inline V8<R> flux_hlld(const Iface<R>& s, R Bn, R gamma) {
         }
         F.c[i] = v;
     }
+    return F;
+}
+
+// ---- Osher-Solomon flux with the Dumbser-Toro path integral (solvers.py:236-284)
+namespace dots {
+// 3-point Gauss-Legendre on [0, 1] as the reference builds them
+// (generic.py:246-248: numpy leggauss(3), x/2 + 1/2, w/2), bit for bit
+__device__ constexpr double ROOT[3] = {0x1.cda042f0236e0p-4, 0x1.0000000000000p-1, 0x1.c64bf7a1fb924p-1};
+__device__ constexpr double WEIGHT[3] = {0x1.1c71c71c71c73p-2, 0x1.c71c71c71c71cp-2, 0x1.1c71c71c71c73p-2};
+
+__device__ inline double sign1(double x) {  // np.sign with 0 -> 1 (constructor.py:198-199)
+    return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : (x == 0.0 ? 1.0 : x));
+}
+
+// wave speeds of a state read as (rho, v, p, B): sound, Alfven along Bn,
+// fast, slow (constructor.py:189-194, solvers.py:252-257)
+struct Speeds {
+    double cs, cax, cf, csl;
+};
+__device__ inline Speeds speeds(const double g[8], double Bn, double gamma) {
+    const double rho = g[0], p = g[4];
+    Speeds w;
+    w.cs = ah_sqrt(gamma * gdiv(p, rho));
+    const double ca = gdiv(ah_sqrt((g[5] * g[5] + g[6] * g[6]) + g[7] * g[7]), ah_sqrt(rho));
+    w.cax = gdiv(Bn, ah_sqrt(rho));
+    const double cs2 = w.cs * w.cs, ca2 = w.cax * w.cax, s = cs2 + ca * ca;
+    const double disc = ah_sqrt(s * s - 4.0 * cs2 * ca2);
+    w.cf = ah_sqrt(0.5 * (s + disc));
+    w.csl = ah_sqrt(0.5 * (s - disc));
+    return w;
+}
+
+// Right eigenvectors of adiabatic MHD at g (constructor.py:180-290), column
+// major: Rc[k][i] = R[i][k].
+__device__ inline void right_eigenvectors(int axis, const double g[8], double gamma, double Rc[8][8]) {
+    const int a = axis % 3, o = (axis + 1) % 3, c = (axis + 2) % 3;
+    const double rho = g[0];
+    const double vx = g[1 + a], vy = g[1 + o], vz = g[1 + c];
+    const double Bx = g[5 + a], By = g[5 + o], Bz = g[5 + c];
+    const Speeds w = speeds(g, Bx, gamma);
+    const double cs = w.cs, cf = w.cf, csl = w.csl;
+    const double S = sign1(Bx);
+    const double cs2 = cs * cs, cf2 = cf * cf, csl2 = csl * csl;
+    const double af = ah_sqrt(gdiv(cs2 - csl2, cf2 - csl2));
+    const double als = ah_sqrt(gdiv(cf2 - cs2, cf2 - csl2));
+    const double bt2 = By * By + Bz * Bz;
+    const double bperp = ah_sqrt(gdiv(bt2, rho));
+    const double b2 = gdiv(By, ah_sqrt(bt2)), b3 = gdiv(Bz, ah_sqrt(bt2));
+    const double vn = ah_sqrt((g[1] * g[1] + g[2] * g[2]) + g[3] * g[3]);
+    const double v2 = vn * vn;
+    const double tr = vy * b2 + vz * b3;
+    const double base_s = (0.5 * als * rho * v2 - cs * af * rho * bperp) + (als * rho * cs2) / (gamma - 1.0);
+    const double ts4 = als * csl * rho * vx, ts5 = af * cf * rho * S * tr;
+    const double base_f = (0.5 * af * rho * v2 + cs * als * rho * bperp) + (af * rho * cs2) / (gamma - 1.0);
+    const double tf4 = af * cf * rho * vx, tf5 = als * csl * rho * S * tr;
+    const double r15 = pow(rho, 1.5), sr = ah_sqrt(rho);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Rc[k][i] = 0.0;
+    // fast -/+ (columns 0, 7)
+    Rc[0][0] = rho * af;
+    Rc[0][a + 1] = rho * af * (vx - cf);
+    Rc[0][o + 1] = rho * (af * vy + als * csl * b2 * S);
+    Rc[0][c + 1] = rho * (af * vz + als * csl * b3 * S);
+    Rc[0][4] = (base_f - tf4) + tf5;
+    Rc[0][o + 5] = als * cs * b2 * sr;
+    Rc[0][c + 5] = als * cs * b3 * sr;
+    Rc[7][0] = rho * af;
+    Rc[7][a + 1] = rho * af * (vx + cf);
+    Rc[7][o + 1] = rho * (af * vy - als * csl * b2 * S);
+    Rc[7][c + 1] = rho * (af * vz - als * csl * b3 * S);
+    Rc[7][4] = (base_f + tf4) - tf5;
+    Rc[7][o + 5] = als * cs * b2 * sr;
+    Rc[7][c + 5] = als * cs * b3 * sr;
+    // slow -/+ (columns 2, 5)
+    Rc[2][0] = rho * als;
+    Rc[2][a + 1] = rho * als * (vx - csl);
+    Rc[2][o + 1] = rho * (als * vy - af * cf * b2 * S);
+    Rc[2][c + 1] = rho * (als * vz - af * cf * b3 * S);
+    Rc[2][4] = (base_s - ts4) - ts5;
+    Rc[2][o + 5] = -af * cs * b2 * sr;
+    Rc[2][c + 5] = -af * cs * b3 * sr;
+    Rc[5][0] = rho * als;
+    Rc[5][a + 1] = rho * als * (vx + csl);
+    Rc[5][o + 1] = rho * (als * vy + af * cf * b2 * S);
+    Rc[5][c + 1] = rho * (als * vz + af * cf * b3 * S);
+    Rc[5][4] = (base_s + ts4) + ts5;
+    Rc[5][o + 5] = -af * cs * b2 * sr;
+    Rc[5][c + 5] = -af * cs * b3 * sr;
+    // Alfven -/+ (columns 1, 6)
+    Rc[1][o + 1] = -b3 * r15;
+    Rc[1][c + 1] = b2 * r15;
+    Rc[1][4] = (b2 * vz - b3 * vy) * r15;
+    Rc[1][o + 5] = -rho * b3;
+    Rc[1][c + 5] = rho * b2;
+    Rc[6][o + 1] = b3 * r15;
+    Rc[6][c + 1] = -b2 * r15;
+    Rc[6][4] = (b3 * vy - b2 * vz) * r15;
+    Rc[6][o + 5] = -rho * b3;
+    Rc[6][c + 5] = rho * b2;
+    // entropy and divergence waves (columns 3, 4)
+    Rc[3][0] = 1.0;
+    Rc[3][a + 1] = vx;
+    Rc[3][o + 1] = vy;
+    Rc[3][c + 1] = vz;
+    Rc[3][4] = 0.5 * v2;
+    Rc[4][4] = Bx;
+    Rc[4][a + 5] = 1.0;
+}
+
+// One-sided (Hestenes) Jacobi SVD of the 8x8 matrix with columns A[k]:
+// on return the columns of A are orthogonal (A_final = A V) and V holds the
+// accumulated rotations, so A = U diag(sigma) V^T with sigma_k = |A_final[k]|.
+__device__ inline void jacobi_svd8(double A[8][8], double V[8][8]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) V[k][i] = i == k ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        bool rotated = false;
+#pragma unroll
+        for (int p = 0; p < 7; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 8; ++q) {
+                double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    al += A[p][i] * A[p][i];
+                    be += A[q][i] * A[q][i];
+                    ga += A[p][i] * A[q][i];
+                }
+                if (!(ah_fabs(ga) > 1e-15 * ah_sqrt(al * be))) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (ah_fabs(zeta) + ah_sqrt(1.0 + zeta * zeta));
+                const double cs = 1.0 / ah_sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const double x = A[p][i], y = A[q][i];
+                    A[p][i] = cs * x - sn * y;
+                    A[q][i] = sn * x + cs * y;
+                    const double u = V[p][i], v = V[q][i];
+                    V[p][i] = cs * u - sn * v;
+                    V[q][i] = sn * u + cs * v;
+                }
+            }
+        if (!rotated) break;
+    }
+}
+}  // namespace dots
+
+// Osher-Solomon flux (solvers.py:236-284).  The path psi(s) = q- + s (q+ - q-)
+// through the conservative states is handed to the eigenvector routine and the
+// wave speeds as if primitive (244-252); the eigenvalues use component 1 and
+// B_x whatever the axis (251-266), the eigenvectors the solver axis.
+// |A| = R |Lambda| pinv(R) (271) with pinv from the SVD, singular values at
+// or below 1e-15 sigma_max dropped as numpy's pinv does; the flux is
+// (F+ + F-)/2 - (J q+ - J q-)/2 with J the weighted sum over the nodes.
+// All 8 components enter, as in the reference (dead ones are zero).
+template <int MASK>
+__device__ inline V8<double> flux_dots(const Iface<double>& s, int sax, double gamma) {
+    double xp[8], xm[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) xp[i] = xm[i] = 0.0;
+    for (int node = 0; node < 3; ++node) {
+        double g[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) g[i] = s.qm.c[i] + dots::ROOT[node] * (s.qp.c[i] - s.qm.c[i]);
+        double A[8][8], V[8][8];
+        dots::right_eigenvectors(sax, g, gamma, A);
+        const dots::Speeds w = dots::speeds(g, g[5], gamma);
+        const double vx = g[1];
+        const double lam[8] = {ah_fabs(vx - w.cf),  ah_fabs(vx - w.cax), ah_fabs(vx - w.csl), ah_fabs(vx),
+                               ah_fabs(vx),         ah_fabs(vx + w.csl), ah_fabs(vx + w.cax), ah_fabs(vx + w.cf)};
+        dots::jacobi_svd8(A, V);
+        double sig2[8], smax = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double n = 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) n += A[k][i] * A[k][i];
+            sig2[k] = n;
+            smax = nmax(smax, ah_sqrt(n));
+        }
+        // pinv(R) q = V diag(1/sigma^2) A_final^T q, then |A| q = R |Lambda| (that)
+        double yp[8], ym[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool keep = ah_sqrt(sig2[k]) > 1e-15 * smax;
+            double dp = 0.0, dm = 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                dp += A[k][i] * s.qp.c[i];
+                dm += A[k][i] * s.qm.c[i];
+            }
+            yp[k] = keep ? dp / sig2[k] : 0.0;
+            ym[k] = keep ? dm / sig2[k] : 0.0;
+        }
+        double zp[8], zm[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            double ap = 0.0, am = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ap += V[k][j] * yp[k];
+                am += V[k][j] * ym[k];
+            }
+            zp[j] = lam[j] * ap;
+            zm[j] = lam[j] * am;
+        }
+        dots::right_eigenvectors(sax, g, gamma, A);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            double ap = 0.0, am = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ap += A[k][i] * zp[k];
+                am += A[k][i] * zm[k];
+            }
+            xp[i] += dots::WEIGHT[node] * ap;
+            xm[i] += dots::WEIGHT[node] * am;
+        }
+    }
+    V8<double> F = zero8<double, MASK>();
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (VS<MASK>::live(i)) F.c[i] = 0.5 * (s.fp.c[i] + s.fm.c[i]) - 0.5 * (xp[i] - xm[i]);
     return F;
 }
 

@@ -97,16 +97,22 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
         if (mask == 0x13) {
             if (solver == LLF) return by_recon<R, 0x13, LLF, 1, SPLIT>(recon, ks);
             if (solver == HLLC) return by_recon<R, 0x13, HLLC, 1, SPLIT>(recon, ks);
+            if constexpr (!SPLIT)
+                if (solver == DOTS) return by_recon<R, 0x13, DOTS, 1, SPLIT>(recon, ks);
         }
         if (mask == 0xFF) {
             if (solver == LLF) return by_recon<R, 0xFF, LLF, 1, SPLIT>(recon, ks);
             if (solver == HLLC) return by_recon<R, 0xFF, HLLC, 1, SPLIT>(recon, ks);
             if (solver == HLLD) return by_recon<R, 0xFF, HLLD, 1, SPLIT>(recon, ks);
+            if constexpr (!SPLIT)
+                if (solver == DOTS) return by_recon<R, 0xFF, DOTS, 1, SPLIT>(recon, ks);
         }
     } else {
         if (mask == 0x17) {
             if (solver == LLF) return by_recon<R, 0x17, LLF, 2, SPLIT>(recon, ks);
             if (solver == HLLC) return by_recon<R, 0x17, HLLC, 2, SPLIT>(recon, ks);
+            if constexpr (!SPLIT)
+                if (solver == DOTS) return by_recon<R, 0x17, DOTS, 2, SPLIT>(recon, ks);
         }
         if (mask == 0x77) {
             if (solver == LLF) return by_recon<R, 0x77, LLF, 2, SPLIT>(recon, ks);

@@ -38,9 +38,11 @@ PARITY = {
     "roe_average": "closed form + golden via steps",
     "limiters.slope (minmod..superbee)": "textbook phi(r) closed forms",
     "extrapolant_limiter mc": "golden limiter_mc.npz",
+    "Osher-Solomon (DOTS) flux + right eigenvectors": "golden schemes_1d.npz *_flux_os (bitwise, 24 cases)",
     "pcm/plm/ppm/weno interface states": "golden schemes_1d.npz",
     "spectral radius (closed)": "golden localmax (LAPACK) to 1e-13 rel",
-    "llf/hllc/hlld/es/os fluxes": "golden schemes_1d.npz",
+    "llf/hllc/hlld fluxes": "golden schemes_1d.npz",
+    "es flux": "not restated: the reference's entropy-stable flux is non-finite on every golden input",
     "lw flux": "parity unpinned (reference picks a LAPACK-order-dependent column)",
     "2D double solve + transverse correction": "golden space_2d.npz",
     "MHD constrained transport": "golden space_2d.npz + steps ot16_*",
@@ -678,15 +680,118 @@ def _fast_slow(rho, p, B, Bn_comp, gamma):
     return cs, ca, cax, np.sqrt(.5 * (cs ** 2 + ca ** 2 + disc)), np.sqrt(.5 * (cs ** 2 + ca ** 2 - disc))
 
 
-def right_eigenvectors(axis, w, gamma):
-    """constructor.py:180-290 (Derigs et al. right eigenvectors, primitive
-    states). Returns (..., 8, 8)."""
-    raise NotImplementedError("DOTS eigenvectors are not restated")
+def right_eigenvectors(axis, g, gamma):
+    """constructor.py:180-290: right eigenvectors of adiabatic MHD (Derigs et
+    al.) evaluated on the states ``g`` (..., 8) read as (rho, v, p, B);
+    returns (..., 8, 8), column k = wave k in the order fast-, Alfven-,
+    slow-, entropy, divergence, slow+, Alfven+, fast+."""
+    a, o, c = axis % 3, (axis + 1) % 3, (axis + 2) % 3
+    rho, vel, p, B = g[..., 0], g[..., 1:4], g[..., 4], g[..., 5:8]
+    vx, vy, vz = vel[..., a], vel[..., o], vel[..., c]
+    Bx, By, Bz = B[..., a], B[..., o], B[..., c]
+    cs, _, _, cf, csl = _fast_slow(rho, p, B, Bx, gamma)
+    S = np.sign(Bx)
+    S[S == 0] = 1
+    af = np.sqrt(gdiv(cs ** 2 - csl ** 2, cf ** 2 - csl ** 2))
+    als = np.sqrt(gdiv(cf ** 2 - cs ** 2, cf ** 2 - csl ** 2))
+    bperp = np.sqrt(gdiv(By ** 2 + Bz ** 2, rho))
+    b2 = gdiv(By, np.sqrt(By ** 2 + Bz ** 2))
+    b3 = gdiv(Bz, np.sqrt(By ** 2 + Bz ** 2))
+    v2 = norm2(vel)
+    tr = vy * b2 + vz * b3
+    # psi_{+-} (constructor.py:206-233): each a left-to-right sum of five terms
+    base_s = (.5 * als * rho * v2 - cs * af * rho * bperp) + (als * rho * cs ** 2) / (gamma - 1)
+    ts4, ts5 = als * csl * rho * vx, af * cf * rho * S * tr
+    base_f = (.5 * af * rho * v2 + cs * als * rho * bperp) + (af * rho * cs ** 2) / (gamma - 1)
+    tf4, tf5 = af * cf * rho * vx, als * csl * rho * S * tr
+    r15, sr = rho ** 1.5, np.sqrt(rho)
+    R = np.zeros(g.shape + (8,))
+    # fast -/+ (columns 0 and 7, constructor.py:237-244, 275-282)
+    for k, plus in ((0, False), (7, True)):
+        R[..., 0, k] = rho * af
+        R[..., o + 5, k] = als * cs * b2 * sr
+        R[..., c + 5, k] = als * cs * b3 * sr
+        if plus:
+            R[..., a + 1, k] = rho * af * (vx + cf)
+            R[..., o + 1, k] = rho * (af * vy - als * csl * b2 * S)
+            R[..., c + 1, k] = rho * (af * vz - als * csl * b3 * S)
+            R[..., 4, k] = (base_f + tf4) - tf5
+        else:
+            R[..., a + 1, k] = rho * af * (vx - cf)
+            R[..., o + 1, k] = rho * (af * vy + als * csl * b2 * S)
+            R[..., c + 1, k] = rho * (af * vz + als * csl * b3 * S)
+            R[..., 4, k] = (base_f - tf4) + tf5
+    # slow -/+ (columns 2 and 5, constructor.py:251-258, 267-274)
+    for k, plus in ((2, False), (5, True)):
+        R[..., 0, k] = rho * als
+        R[..., o + 5, k] = -af * cs * b2 * sr
+        R[..., c + 5, k] = -af * cs * b3 * sr
+        if plus:
+            R[..., a + 1, k] = rho * als * (vx + csl)
+            R[..., o + 1, k] = rho * (als * vy + af * cf * b2 * S)
+            R[..., c + 1, k] = rho * (als * vz + af * cf * b3 * S)
+            R[..., 4, k] = (base_s + ts4) + ts5
+        else:
+            R[..., a + 1, k] = rho * als * (vx - csl)
+            R[..., o + 1, k] = rho * (als * vy - af * cf * b2 * S)
+            R[..., c + 1, k] = rho * (als * vz - af * cf * b3 * S)
+            R[..., 4, k] = (base_s - ts4) - ts5
+    # Alfven -/+ (constructor.py:245-250, 267-272)
+    R[..., o + 1, 1], R[..., c + 1, 1] = -b3 * r15, b2 * r15
+    R[..., 4, 1] = (b2 * vz - b3 * vy) * r15
+    R[..., o + 1, 6], R[..., c + 1, 6] = b3 * r15, -b2 * r15
+    R[..., 4, 6] = (b3 * vy - b2 * vz) * r15
+    for k in (1, 6):
+        R[..., o + 5, k] = -rho * b3
+        R[..., c + 5, k] = rho * b2
+    # entropy and divergence waves (constructor.py:259-266)
+    R[..., 0, 3] = 1
+    R[..., a + 1, 3], R[..., o + 1, 3], R[..., c + 1, 3] = vx, vy, vz
+    R[..., 4, 3] = .5 * v2
+    R[..., 4, 4] = Bx
+    R[..., a + 5, 4] = 1
+    return R
+
+
+def gauss_legendre_unit():
+    """generic.py:246-248: 3-point Gauss-Legendre nodes and weights mapped to
+    [0, 1] (numpy leggauss, then x/2 + 1/2 and w/2)."""
+    r, w = np.polynomial.legendre.leggauss(3)
+    return .5 * r + .5, w / 2
+
+
+def flux_dots(ax, gamma, d):
+    """solvers.py:236-284: Osher-Solomon flux with the Dumbser-Toro path
+    integral.  The path psi(s) = q- + s (q+ - q-) runs through the
+    conservative states and is handed to the eigenvector routine and the wave
+    speeds as if primitive (solvers.py:244-252); the speeds use index 1 and
+    B[..., 0] whatever the axis (251-256), the eigenvectors the axis.
+    |A| = R |Lambda| pinv(R) (271) with numpy's SVD pseudo-inverse (singular
+    values <= 1e-15 sigma_max dropped), summed with the quadrature weights."""
+    qp, qm = d["qFs"]
+    fp, fm = d["fluxFs"]
+    ax %= 3
+    roots, weights = gauss_legendre_unit()
+    P = np.repeat(qp[None], len(roots), axis=0)
+    M = np.repeat(qm[None], len(roots), axis=0)
+    psi = M + (roots * (P - M).T).T
+    R = right_eigenvectors(ax, psi, gamma)
+    rho, vx, p, B = psi[..., 0], psi[..., 1], psi[..., 4], psi[..., 5:8]
+    _, _, cax, cf, csl = _fast_slow(rho, p, B, B[..., 0], gamma)
+    lam = np.zeros_like(R)
+    for k, v in enumerate((vx - cf, vx - cax, vx - csl, vx, vx, vx + csl, vx + cax, vx + cf)):
+        lam[..., k, k] = v
+    absA = R @ np.abs(lam) @ np.linalg.pinv(R)
+    J = np.sum((weights * absA.T).T, axis=0)
+    dq_p = (J @ qp[..., None])[..., 0]
+    dq_m = (J @ qm[..., None])[..., 0]
+    return .5 * (fp + fm) - .5 * (dq_p - dq_m)
 
 
 def flux_es(ax, gamma, d):
-    """solvers.py:288-391 entropy-stable flux: restated only for its
-    entropy-conserving part; parity unpinned."""
+    """solvers.py:288-391 entropy-stable flux: not restated.  The reference's
+    version averages with a half-difference (arith_mean, 306-307) and returns
+    non-finite fluxes on every golden input (schemes_1d.npz *_flux_es)."""
     raise NotImplementedError("ES flux is not restated")
 
 
@@ -699,6 +804,8 @@ def riemann(s, ax, lmax, chars, d):
             return flux_hllc(ax, s.gamma, d)
         return flux_hllc(ax, s.gamma, d, ax + 1, default_minus=True)
     if s.solver_category == "complete":
+        if s.solver.startswith("o"):
+            return flux_dots(ax, s.gamma, d)
         return flux_es(ax, s.gamma, d)
     if s.solver.endswith("w"):
         return flux_lw(chars, d)

@@ -60,6 +60,12 @@ def test_solver_routing():
     assert O.solver_of("lax-wendroff") == L.AH_LLF     # solvers.py:28: only names ending in 'w' reach LW
     with pytest.raises(NotImplementedError):
         O.solver_of("lw")
+    # solvers.py:20-24: "complete" names starting with "o" are Osher-Solomon, the rest entropy-stable
+    for name in ("os", "osher", "osher-solomon", "osher solomon"):
+        assert O.solver_of(name) == L.AH_DOTS
+    for name in ("solomon", "es", "entropy", "entropy-stable"):
+        with pytest.raises(NotImplementedError):
+            O.solver_of(name)
 
 
 def test_invalid_values_revert_to_defaults():

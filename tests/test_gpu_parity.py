@@ -92,6 +92,77 @@ def test_interface_flux_1d(key, solver):
     assert abs(fl[0]["eigmax"] - float(g[key + "_eigmax"])) <= 1e-14 * float(g[key + "_eigmax"])
 
 
+DOTS_KEYS = [f"{p}_{sg}_{bc}" for p in ("hd", "mhd") for sg in ("pcm", "plm", "ppm", "weno3", "weno5", "weno7")
+             for bc in ("edge", "wrap")]
+
+
+@pytest.mark.parametrize("key", DOTS_KEYS)
+def test_dots_interface_flux_1d(key):
+    """Osher-Solomon interface fluxes (solvers.py:236-284) against the
+    reference's (schemes_1d.npz *_flux_os) and the oracle.  The device takes
+    the pseudo-inverse from a Jacobi SVD where the reference calls LAPACK, so
+    agreement is to rounding times the conditioning of the eigenvector
+    matrices, not bit for bit."""
+    from astrea_amd.plan import Plan
+    g = load("schemes_1d.npz")
+    phys, sg, bc = key.split("_")
+    mhd = phys == "mhd"
+    gamma = 5 / 3 if mhd else 1.4
+    grid = g[key + "_grid"]
+    N = grid.shape[0]
+    _say(f"[gpu] dots flux1d {key}")
+    plan = Plan(1, sg, "os", "euler", [dict(n0=N, boundary=bc, dx=1 / 32, gamma=gamma, cfl=.5, t_end=1.)],
+                varmask=0xFF if mhd else 0x13)
+    plan.load([grid])
+    fl = plan.evolve_space(0)
+    plan.close()
+    s = O.Scheme(dimension=1, subgrid=sg, solver="os", boundary=bc, gamma=gamma, dx=1 / 32)
+    d = O.reconstruct(grid, s, 0)
+    F_or = O.riemann(s, 0, None, None, d)
+    F_ref = g[key + "_flux_os"]
+    got = fl[0]["flux"]
+    # per-component scale, floored at 1e-4 of the largest flux: the normal-field
+    # flux is identically zero up to rounding noise (~1e-17) on both sides
+    scale = np.maximum(np.max(np.abs(F_ref), axis=0), 1e-4 * np.max(np.abs(F_ref)))
+    e_or, e_ref = np.max(np.abs(got - F_or) / scale), np.max(np.abs(got - F_ref) / scale)
+    _say(f"[gpu]   max scaled error vs oracle {e_or:.2e}, vs reference {e_ref:.2e}")
+    assert e_or <= 1e-11 and e_ref <= 1e-11
+
+
+@pytest.mark.parametrize("dims,subgrid", [(1, "plm"), (1, "ppm"), (2, "plm")])
+def test_dots_trajectory_vs_oracle(dims, subgrid):
+    """Osher-Solomon runs (1D Riemann problem 64 cells, 2D quadrants 16^2),
+    SSPRK(3,3), against the oracle: same steps, rel L1 per variable."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    if dims == 1:
+        n, steps = 64, 20
+        w = ic.riemann_1d(n, (2.5, .7, 4.), (.3, -.4, .2))
+        prob = dict(n0=n, boundary="edge", dx=1 / n, gamma=1.4, cfl=.5, t_end=1e9, max_steps=steps)
+        vm = 0x13
+    else:
+        n, steps = 16, 4
+        w = ic.quadrants(n, [(1, .2, .1, 1), (.5, -.3, .2, .4), (.2, .1, -.5, .2), (.8, 0, 0, .9)])
+        prob = dict(n0=n, n1=n, boundary="edge", dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9, max_steps=steps)
+        vm = 0x17
+    q0 = ic.initial_average(torch.from_numpy(w), 1.4, "edge", subgrid == "ppm").numpy()
+    _say(f"[gpu] dots trajectory {dims}D {subgrid} n={n} steps={steps}")
+    plan = Plan(dims, subgrid, "os", "ssprk(3,3)", [prob], varmask=vm)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=steps)[0]
+    out = plan.unpack(0)
+    plan.close()
+    sch = O.Scheme(dimension=dims, subgrid=subgrid, solver="os", timestep="ssprk(3,3)", boundary="edge", gamma=1.4,
+                   cfl=prob["cfl"], dx=1 / n)
+    ref = O.run(q0, sch, 1e9, max_steps=steps)
+    assert st["steps"] == steps == ref.steps and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
 @pytest.mark.parametrize("case", [("ppm", "hllc", "wrap"), ("plm", "hllc", "edge"), ("ppm", "llf", "edge")])
 @pytest.mark.parametrize("parity", [0, 1])
 def test_interface_flux_2d(case, parity):

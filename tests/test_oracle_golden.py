@@ -50,7 +50,7 @@ def test_interface_states_and_fluxes(key):
     ref = g[key + "_localmax"]
     assert np.max(np.abs(lm - ref) / np.abs(ref)) < 1e-14
     lm_lapack, _ = O.local_max_eig(d["jac_state"], replace(s, eig="lapack"), 0)
-    for solver in (["llf", "hllc"] if mhd == "hd" else ["llf", "hlld"]):
+    for solver in (["llf", "hllc", "os"] if mhd == "hd" else ["llf", "hlld", "os"]):
         F = O.riemann(replace(s, solver=solver), 0, lm_lapack, None, d)
         assert np.array_equal(F, g[key + f"_flux_{solver}"]), solver
 
