@@ -101,6 +101,7 @@ struct ah_plan {
     void* wU[2] = {nullptr, nullptr};
     double* e3 = nullptr;
     double* scratch = nullptr;
+    double* bsave = nullptr;  // B_x, B_y of the state at the start of the step (MHD)
     bool bound = false;
     // kernel timing (ah_plan_profile)
     std::vector<hipEvent_t> ev;
@@ -293,7 +294,7 @@ int ah_plan_workspace_size(const ah_plan* p, size_t* bytes) {
         b += ((size_t)p->total * p->ks.real_bytes + 255) & ~size_t(255);  // normal field for HLLD
         if (p->sch.magnetic_2d) {
             b += 6 * plane_bytes(p, p->ks.real_bytes);                          // wF, wD, wU per sweep
-            b += 3 * (((size_t)p->total * 8 + 255) & ~size_t(255));            // e3, 2 scratch planes
+            b += 5 * (((size_t)p->total * 8 + 255) & ~size_t(255));            // e3, 2 scratch, 2 B backup planes
         }
     }
     *bytes = b;
@@ -351,6 +352,8 @@ int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
             p->e3 = (double*)c;
             c += ((size_t)p->total * 8 + 255) & ~size_t(255);
             p->scratch = (double*)c;
+            c += 2 * (((size_t)p->total * 8 + 255) & ~size_t(255));
+            p->bsave = (double*)c;
         }
     }
     p->bound = true;
@@ -469,6 +472,7 @@ static MhdArgs mhd_args(ah_plan* p) {
     m.L = p->Lbuf;
     m.U = p->U;
     m.scratch = p->scratch;
+    m.bsave = p->bsave;
     m.plane = p->total;
     m.pd = p->d_pd;
     m.ps = p->d_ps;

@@ -740,37 +740,69 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     __shared__ double red[NTHR / 64];
 
     const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
+    const R* wLg = reinterpret_cast<const R*>(a.wL);
+    const R* wRg = reinterpret_cast<const R*>(a.wR);
+    const long long P = a.plane;
+    // w-/w+ window of a tile, interfaces [s0, s0+TS+1] x lines [t0-2, t0+TT+2),
+    // fetched into registers one tile ahead: the loads of the next tile are in
+    // flight while this tile computes (the LDS windows limit residency to two
+    // workgroups per CU, so the registers are free)
+    constexpr int NA = (NT * NK + NTHR - 1) / NTHR;
+    R pfm[NA][NV], pfp[NA][NV];
+    auto fetch = [&](int ti) {
+        const Tile tl = ld_const(a.tiles + ti);
+        const ProbDesc q = ld_const(a.pd + tl.prob);
+        const int ns_ = AX == 0 ? q.n0 : q.n1, nt_ = AX == 0 ? q.n1 : q.n0;
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+            const int i = threadIdx.x + j * NTHR;
+            if (i < NT * NK) {
+                const int it = i / NK, ik = i % NK;
+                const int k = tl.s0 + ik;
+                const long long row = q.off + (long long)bcmap(tl.t0 - 2 + it, nt_, q.bc) * ns_;
+                const long long om = row + bcmap(k - 1, ns_, q.bc), op = row + bcmap(k, ns_, q.bc);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    pfm[j][v] = wRg[v * P + om];
+                    pfp[j][v] = wLg[v * P + op];
+                }
+            }
+        }
+    };
+    if ((int)blockIdx.x < ntiles_) fetch(blockIdx.x);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
     const Tile tile = ld_const(a.tiles + ti_);
     const ProbDesc pd = ld_const(a.pd + tile.prob);
     ProbState* ps = a.ps + tile.prob;
-    if (!ld_const(&ps->active)) continue;
+    const bool active = ld_const(&ps->active);
+    // A: the prefetched window into LDS, then start the next tile's loads
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+            const int i = threadIdx.x + j * NTHR;
+            if (i < NT * NK) {
+                const int it = i / NK, ik = i % NK;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    wm_s[v][it][ik] = pfm[j][v];
+                    wp_s[v][it][ik] = pfp[j][v];
+                }
+            }
+        }
+    }
+    if (ti_ + (int)gridDim.x < ntiles_) fetch(ti_ + gridDim.x);
+    if (!active) continue;
     const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
     const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
     const R gR = R(pd.gamma);
-    const long long off = pd.off, P = a.plane;
+    const long long off = pd.off;
     const int parity = a.parity_override >= 0 ? a.parity_override : ld_const(&ps->parity);
     // solver axis: the enumeration index of the sweep (solvers.py:34-36), or
     // the physical axis in the physical-compat mode
     const int sax = a.compat ? AX : (parity == 0 ? AX : 1 - AX);
     const int bt = t0 - 2, bta = t0 - 1;
-    const R* wLg = reinterpret_cast<const R*>(a.wL);
-    const R* wRg = reinterpret_cast<const R*>(a.wR);
     auto sidx = [&](int s, int t) { return off + (long long)t * ns + s; };  // sweep-frame layout
     auto gidx = [&](int s, int t) { return off + (AX == 0 ? (long long)s * pd.n1 + t : (long long)t * pd.n1 + s); };
-
-    // A: w-/w+ at interfaces [s0, s0+TS+1] x lines [t0-2, t0+TT+2)
-    for (int i = threadIdx.x; i < NT * NK; i += blockDim.x) {
-        const int it = i / NK, ik = i % NK;
-        const int k = s0 + ik;
-        const int gt = bcmap(bt + it, nt, bc);
-        const long long om = sidx(bcmap(k - 1, ns, bc), gt), op = sidx(bcmap(k, ns, bc), gt);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            wm_s[v][it][ik] = wRg[v * P + om];
-            wp_s[v][it][ik] = wLg[v * P + op];
-        }
-    }
     __syncthreads();
     auto WM = [&](int it, int ik) {
         V8<R> w = zero8<R, MASK>();

@@ -20,6 +20,7 @@ struct MhdArgs {
     double* U;            // state (stage-0 face-field reassignment)
     const double* K;      // register to refine (B planes)
     double* scratch;      // two planes for the refined B
+    double* bsave;        // two planes: B_x, B_y of the state before the stage-0 reassignment
     long long plane;
     const ProbDesc* pd;
     const ProbState* ps;
@@ -208,16 +209,23 @@ __global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
     const R* Fx = reinterpret_cast<const R*>(a.src[0]);
     const R* Fy = reinterpret_cast<const R*>(a.src[1]);
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = a.tiles[ti];
+        const Tile tl = ld_const(a.tiles + ti);
         const ProbState& st = a.ps[tl.prob];
-        if (!st.active || st.fail_flag) continue;
-        const ProbDesc pd = a.pd[tl.prob];
+        if (!ld_const(&st.active)) continue;
+        // the state's B is kept so that a step that fails in a later stage
+        // can hand back the last good state (copyB_kernel restores it)
+        const bool failed = ld_const(&st.fail_flag) != 0;
+        const ProbDesc pd = ld_const(a.pd + tl.prob);
         const int n0 = pd.n0, n1 = pd.n1;
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
             const int cell = tl.s0 + c;
             const int i = cell / n1, j = cell % n1;
-            a.U[V::slot(5) * P + pd.off + cell] = double(Fx[V::slot(5) * P + pd.off + (long long)j * n0 + i]);
-            a.U[V::slot(6) * P + pd.off + cell] = double(Fy[V::slot(6) * P + pd.off + (long long)i * n1 + j]);
+            const long long ox = V::slot(5) * P + pd.off + cell, oy = V::slot(6) * P + pd.off + cell;
+            a.bsave[pd.off + cell] = a.U[ox];
+            a.bsave[P + pd.off + cell] = a.U[oy];
+            if (failed) continue;
+            a.U[ox] = double(Fx[V::slot(5) * P + pd.off + (long long)j * n0 + i]);
+            a.U[oy] = double(Fy[V::slot(6) * P + pd.off + (long long)i * n1 + j]);
         }
     }
 }
@@ -275,14 +283,21 @@ __global__ __launch_bounds__(256) void copyB_kernel(MhdArgs a, double* dst) {
     const int ntiles = wave_uniform(*a.ntiles_dev);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = a.tiles[ti];
+        const Tile tl = ld_const(a.tiles + ti);
         const ProbState& st = a.ps[tl.prob];
-        if (!st.active || st.fail_flag) continue;
-        const ProbDesc pd = a.pd[tl.prob];
+        if (!ld_const(&st.active)) continue;
+        const bool failed = ld_const(&st.fail_flag) != 0;
+        const ProbDesc pd = ld_const(a.pd + tl.prob);
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
             const long long o = pd.off + tl.s0 + c;
-            dst[V::slot(5) * P + o] = a.scratch[o];
-            dst[V::slot(6) * P + o] = a.scratch[P + o];
+            if (failed) {
+                // the step failed: the state keeps the B it had when the step began
+                a.U[V::slot(5) * P + o] = a.bsave[o];
+                a.U[V::slot(6) * P + o] = a.bsave[P + o];
+            } else {
+                dst[V::slot(5) * P + o] = a.scratch[o];
+                dst[V::slot(6) * P + o] = a.scratch[P + o];
+            }
         }
     }
 }

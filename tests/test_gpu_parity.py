@@ -512,3 +512,38 @@ def test_mhd_longer_run_vs_oracle():
     e = np.max(rel_l1(out, ref.grid))
     _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
     assert e <= 1e-12
+
+
+def test_mhd_orszag_tang_follows_reference_to_failure():
+    """Orszag-Tang 64^2, WENO5 + HLLD + constrained transport: the reference
+    algorithm reaches a non-finite state at step 17.  The device equals the
+    oracle bit for bit through step 16 and raises the failure flag at step
+    17, keeping the last finite state (where the reference's LAPACK call
+    would raise)."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    n = 64
+    q0 = ic.initial_average(torch.from_numpy(ic.orszag_tang(n)), 5 / 3, "wrap", True).numpy()
+    _say("[gpu] mhd OT 64^2 weno5/hlld to failure")
+    plan = Plan(2, "weno5", "hlld", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=5 / 3, cfl=.4,
+                                                          t_end=1e9, max_steps=40)], varmask=0x77, magnetic_2d=True,
+                compat="physical")
+    plan.load([q0])
+    plan.reset()
+    plan.step(16)
+    st16 = plan.status()[0]
+    out16 = plan.unpack(0)
+    plan.step(24)
+    st = plan.status()[0]
+    out_last = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid="weno5", solver="hlld", timestep="ssprk(3,3)", boundary="wrap", gamma=5 / 3,
+                 cfl=.4, dx=1 / n, magnetic_2d=True, compat="physical")
+    r16 = O.run(q0, s, 1e9, max_steps=16)
+    r17 = O.run(q0, s, 1e9, max_steps=17)
+    assert st16["steps"] == 16 and not st16["failed"]
+    assert np.array_equal(out16, r16.grid)
+    assert not np.isfinite(r17.grid).all()
+    assert st["failed"] and st["steps"] == 16
+    assert np.array_equal(out_last, out16)
