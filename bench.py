@@ -53,7 +53,9 @@ def parse():
     ap.add_argument("--nprob", type=int, default=512)
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
-    ap.add_argument("--rate", type=float, default=0.4, help="serve: arrivals per second")
+    ap.add_argument("--rate", type=float, default=3.0,
+                    help="serve: arrivals per second (3.0/s loads the plain FP64 loop to ~83%%: the stream "
+                         "averages 2.4e8 zone-updates per request at ~8.6e8 ZU/s)")
     return ap.parse_args()
 
 
