@@ -18,7 +18,7 @@ namespace ah {
 
 enum { EDGE = 0, WRAP = 1 };
 enum { PCM = 0, PLM = 1, PPM = 2, WENO3 = 3, WENO5 = 4, WENO7 = 5 };
-enum { LLF = 0, HLLC = 1, HLLD = 2, DOTS = 3 };
+enum { LLF = 0, HLLC = 1, HLLD = 2, DOTS = 3, ES = 4 };
 enum { MINMOD = 0, VANLEER = 1, OSPRE = 2, VANALBADA1 = 3, KOREN = 4, SUPERBEE = 5 };
 enum { FLAT_OFF = 0, FLAT_ASTREA = 1, FLAT_COLELLA90 = 2 };
 
@@ -919,6 +919,137 @@ __device__ inline V8<double> flux_dots(const Iface<double>& s, int sax, double g
 #pragma unroll
     for (int i = 0; i < 8; ++i)
         if (VS<MASK>::live(i)) F.c[i] = 0.5 * (s.fp.c[i] + s.fm.c[i]) - 0.5 * (xp[i] - xm[i]);
+    return F;
+}
+
+// ---- entropy-stable flux (solvers.py:288-391), 1D as in the reference --------
+namespace esf {
+__device__ inline double am(double l, double r) { return 0.5 * (l - r); }  // "arith_mean" (306-307)
+__device__ inline double glog(double x) { return x > 0.0 ? log(x) : 0.0; }   // fv.log (fv.py:25-26)
+__device__ inline double lon(double l, double r) { return gdiv(l - r, glog(l) - glog(r)); }
+}  // namespace esf
+
+// Entropy-conserving flux (Winters & Gassner) plus the hybrid Roe/LLF
+// dissipation R |Lambda| R^T (entropy jump) of Derigs et al., as the
+// reference writes them: the "arithmetic mean" is the half-difference
+// .5 (L - R), velocity and field components are taken at indices 0, 1, 2
+// whatever the axis, the eigenvectors (constructor.py:294-415) at the
+// averaged state with the axis.  Every matrix product is evaluated term by
+// term, so a non-finite entry spreads along its row exactly as in the
+// reference's matmuls (most of its outputs are non-finite).
+template <int MASK>
+__device__ inline V8<double> flux_es(const Iface<double>& s, int sax, double gamma) {
+    using esf::am;
+    using esf::lon;
+    const int a = sax % 3, o = (sax + 1) % 3, c = (sax + 2) % 3;
+    const V8<double>& L = s.wm;
+    const V8<double>& Rr = s.wp;
+    const double rhoL = L.c[0], PL = L.c[4], rhoR = Rr.c[0], PR = Rr.c[4];
+    const double z1L = ah_sqrt(gdiv(rhoL, PL)), z1R = ah_sqrt(gdiv(rhoR, PR));
+    const double z5L = ah_sqrt(rhoL * PL), z5R = ah_sqrt(rhoR * PR);
+    const double rho_h = am(z1L, z1R) * lon(z5L, z5R);
+    const double P1 = gdiv(am(z5L, z5R), am(z1L, z1R));
+    const double P2 = ((gamma + 1.0) / (2.0 * gamma)) * (gdiv(lon(z5L, z5R), lon(z1L, z1R))) +
+                      ((gamma - 1.0) / (2.0 * gamma)) * (gdiv(am(z5L, z5R), am(z1L, z1R)));
+    double hv1[3], hv2[3], Bh[3], Bd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double vl = L.c[1 + k], vr = Rr.c[1 + k];
+        hv1[k] = gdiv(am(vl * z1L, vr * z1R), am(z1L, z1R));
+        hv2[k] = gdiv(am(vl * (z1L * z1L), vr * (z1R * z1R)), am(z1L * z1L, z1R * z1R));
+        Bh[k] = am(L.c[5 + k], Rr.c[5 + k]);
+        Bd[k] = am(L.c[5 + k] * L.c[5 + k], Rr.c[5 + k] * Rr.c[5 + k]);
+    }
+    const double B1B2 = am(L.c[5] * L.c[6], Rr.c[5] * Rr.c[6]), B1B3 = am(L.c[5] * L.c[7], Rr.c[5] * Rr.c[7]);
+    const double u1 = hv1[0], v1 = hv1[1], w1 = hv1[2], u2 = hv2[0], v2 = hv2[1], w2 = hv2[2];
+    double ec[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ec[i] = 0.0;
+    ec[0] = rho_h * u1;
+    ec[a + 1] = P1 + rho_h * (u1 * u1) + 0.5 * ((Bd[0] + Bd[1]) + Bd[2]) - Bd[0];
+    ec[o + 1] = rho_h * u1 * v1 - B1B2;
+    ec[c + 1] = rho_h * u1 * w1 - B1B3;
+    ec[4] = (gamma / (gamma - 1.0)) * u1 * P2 + 0.5 * rho_h * u1 * ((u1 * u1 + v1 * v1) + w1 * w1) +
+            u2 * (Bh[1] * Bh[1] + Bh[2] * Bh[2]) - Bh[0] * (v2 * Bh[1] + w2 * Bh[2]);
+    ec[o + 5] = u2 * Bh[1] - v2 * Bh[0];
+    ec[c + 5] = u2 * Bh[2] - w2 * Bh[0];
+    // eigenvectors at the averaged state, then R = R0 @ sqrt(D) (constructor.py:404-414)
+    const double hat[8] = {rho_h, u1, v1, w1, P1, Bh[0], Bh[1], Bh[2]};
+    double R0[8][8];  // column major
+    dots::right_eigenvectors(sax, hat, gamma, R0);
+    double Dd[8];
+    Dd[0] = 1.0 / (2.0 * gamma * rho_h);
+    Dd[a + 1] = gdiv(P1, 2.0 * (rho_h * rho_h));
+    Dd[o + 1] = 1.0 / (2.0 * gamma * rho_h);
+    Dd[c + 1] = (rho_h * (gamma - 1.0)) / gamma;
+    Dd[4] = gdiv(P1, rho_h);
+    Dd[a + 5] = 1.0 / (2.0 * gamma * rho_h);
+    Dd[o + 5] = gdiv(P1, 2.0 * (rho_h * rho_h));
+    Dd[c + 5] = 1.0 / (2.0 * gamma * rho_h);
+    double Rm[8][8];  // row major: Rm[i][j] = sum_k R0[i][k] sqrt(D)[k][j]
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc += R0[k][i] * (k == j ? ah_sqrt(Dd[j]) : 0.0);
+            Rm[i][j] = acc;
+        }
+    // wave speeds at the averaged state and the hybrid eigenvalue matrix
+    double Bhat[8] = {0.0, 0.0, 0.0, 0.0, 0.0, Bh[0], Bh[1], Bh[2]};
+    Bhat[0] = rho_h;
+    Bhat[4] = P1;
+    const dots::Speeds w = dots::speeds(Bhat, Bh[0], gamma);
+    const double wv[8] = {u1 + w.cf, u1 + w.cax, u1 + w.csl, u1, u1, u1 - w.csl, u1 - w.cax, u1 - w.cf};
+    double mx = ah_fabs(wv[0]);
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+        if (k != 4) {
+            const double x = ah_fabs(wv[k]);
+            mx = (x > mx || x != x) ? x : mx;  // np.maximum.reduce: NaN propagates
+        }
+    const double eps = ah_sqrt(ah_fabs(gdiv(PR - PL, PR + PL)));
+    const double offd = (1.0 - eps) * 0.0 + eps * 0.0;
+    double lamd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lamd[k] = (1.0 - eps) * ah_fabs(wv[k]) + eps * mx;
+    // entropy-variable jump (solvers.py:371-377), negated
+    const double nL = ah_sqrt((L.c[1] * L.c[1] + L.c[2] * L.c[2]) + L.c[3] * L.c[3]);
+    const double nR = ah_sqrt((Rr.c[1] * Rr.c[1] + Rr.c[2] * Rr.c[2]) + Rr.c[3] * Rr.c[3]);
+    double e[8];
+    e[0] = ((gamma - log(PL * pow(rhoL, -gamma))) / (gamma - 1.0) - gdiv(0.5 * rhoL * (nL * nL), PL)) -
+           ((gamma - log(PR * pow(rhoR, -gamma))) / (gamma - 1.0) - gdiv(0.5 * rhoR * (nR * nR), PR));
+    e[4] = gdiv(rhoR, PR) - gdiv(rhoL, PL);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e[1 + k] = gdiv(L.c[1 + k] * rhoL, PL) - gdiv(Rr.c[1 + k] * rhoR, PR);
+        e[5 + k] = gdiv(L.c[5 + k] * rhoL, PL) - gdiv(Rr.c[5 + k] * rhoR, PR);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = -e[k];
+    // dissipation = (R lam) R^T e, streamed by rows of R lam
+    V8<double> F = zero8<double, MASK>();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        double M[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) acc += Rm[i][m] * (m == k ? lamd[k] : offd);
+            M[k] = acc;
+        }
+        double d = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            double aij = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) aij += M[k] * Rm[j][k];
+            d += aij * e[j];
+        }
+        if (VS<MASK>::live(i)) F.c[i] = ec[i] + 0.5 * d;
+    }
     return F;
 }
 

@@ -7,7 +7,10 @@ namespace ah {
 
 template <class R, int MASK, int RECON, int SOLVER, bool SPLIT>
 inline void set1d(KernelSet* ks) {
-    constexpr int TS = VS<MASK>::NV > 4 ? 128 : 256;
+    // cells per tile: the widest per-stage window (the q window, TS + 13)
+    // fits one pass of the 256 threads, so no stage runs a second, nearly
+    // empty round while the other waves wait at the barrier
+    constexpr int TS = VS<MASK>::NV > 4 ? 128 : 256 - 13;
     ks->sweep1d = [](const SweepArgs& a, int nb, hipStream_t s) {
         launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS, SPLIT>, a, nb, s);
     };
@@ -104,8 +107,12 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
             if (solver == LLF) return by_recon<R, 0xFF, LLF, 1, SPLIT>(recon, ks);
             if (solver == HLLC) return by_recon<R, 0xFF, HLLC, 1, SPLIT>(recon, ks);
             if (solver == HLLD) return by_recon<R, 0xFF, HLLD, 1, SPLIT>(recon, ks);
-            if constexpr (!SPLIT)
+            if constexpr (!SPLIT) {
                 if (solver == DOTS) return by_recon<R, 0xFF, DOTS, 1, SPLIT>(recon, ks);
+                // entropy-stable: 1D only (as the reference), all 8 components, whose
+                // non-finite values feed the reference's failure on the next step
+                if (solver == ES) return by_recon<R, 0xFF, ES, 1, SPLIT>(recon, ks);
+            }
         }
     } else {
         if (mask == 0x17) {

@@ -263,6 +263,8 @@ __device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma
                      : flux_hllc<R, MASK>(s, sax, gamma, physical ? mom : 1, physical);
     else if constexpr (SOLVER == DOTS)
         return flux_dots<MASK>(s, sax, gamma);
+    else if constexpr (SOLVER == ES)
+        return flux_es<MASK>(s, sax, gamma);
     else
         return sax == 0 ? flux_hlld<R, MASK, 0>(s, Bn, gamma) : flux_hlld<R, MASK, 1>(s, Bn, gamma);
 }

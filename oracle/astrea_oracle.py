@@ -42,7 +42,7 @@ PARITY = {
     "pcm/plm/ppm/weno interface states": "golden schemes_1d.npz",
     "spectral radius (closed)": "golden localmax (LAPACK) to 1e-13 rel",
     "llf/hllc/hlld fluxes": "golden schemes_1d.npz",
-    "es flux": "not restated: the reference's entropy-stable flux is non-finite on every golden input",
+    "es flux": "golden schemes_1d.npz *_flux_es (NaN positions included)",
     "lw flux": "parity unpinned (reference picks a LAPACK-order-dependent column)",
     "2D double solve + transverse correction": "golden space_2d.npz",
     "MHD constrained transport": "golden space_2d.npz + steps ot16_*",
@@ -788,11 +788,87 @@ def flux_dots(ax, gamma, d):
     return .5 * (fp + fm) - .5 * (dq_p - dq_m)
 
 
+def es_right_eigenvectors(axis, g, gamma):
+    """constructor.py:294-415: the columns of right_eigenvectors (the same
+    formulas), times the square root of Barth's diagonal scaling (404-414) as
+    a matrix product, so a non-finite entry spreads exactly as there."""
+    a, o, c = axis % 3, (axis + 1) % 3, (axis + 2) % 3
+    R = right_eigenvectors(axis, g, gamma)
+    rho, p = g[..., 0], g[..., 4]
+    D = np.zeros_like(R)
+    D[..., 0, 0] = 1 / (2 * gamma * rho)
+    D[..., a + 1, a + 1] = gdiv(p, 2 * rho ** 2)
+    D[..., o + 1, o + 1] = 1 / (2 * gamma * rho)
+    D[..., c + 1, c + 1] = (rho * (gamma - 1)) / gamma
+    D[..., 4, 4] = gdiv(p, rho)
+    D[..., a + 5, a + 5] = 1 / (2 * gamma * rho)
+    D[..., o + 5, o + 5] = gdiv(p, 2 * rho ** 2)
+    D[..., c + 5, c + 5] = 1 / (2 * gamma * rho)
+    return R @ np.sqrt(D)
+
+
 def flux_es(ax, gamma, d):
-    """solvers.py:288-391 entropy-stable flux: not restated.  The reference's
-    version averages with a half-difference (arith_mean, 306-307) and returns
-    non-finite fluxes on every golden input (schemes_1d.npz *_flux_es)."""
-    raise NotImplementedError("ES flux is not restated")
+    """solvers.py:288-391 entropy-stable flux (Winters & Gassner EC flux plus
+    Derigs et al. hybrid Roe/LLF dissipation), as written: its "arithmetic
+    mean" is the half-difference .5 (L - R) (306-307), the velocity and field
+    components are taken at indices 0, 1, 2 whatever the axis (320-321), the
+    eigenvectors at the averaged state with the axis.  The reference runs it
+    in 1D only (its transpose(0, 2, 1) at 387 needs (N, 8, 8) stacks)."""
+    wp, wm = d["wFs"]
+    a, o, c = ax % 3, (ax + 1) % 3, (ax + 2) % 3
+
+    def am(t):
+        return .5 * (t[0] - t[1])
+
+    def lon(t):
+        return gdiv(t[0] - t[1], glog(t[0]) - glog(t[1]))
+
+    rhoL, vL, PL, BL = wm[..., 0], wm[..., 1:4], wm[..., 4], wm[..., 5:8]
+    rhoR, vR, PR, BR = wp[..., 0], wp[..., 1:4], wp[..., 4], wp[..., 5:8]
+    z1 = np.array([np.sqrt(gdiv(rhoL, PL)), np.sqrt(gdiv(rhoR, PR))])
+    z5 = np.array([np.sqrt(rhoL * PL), np.sqrt(rhoR * PR)])
+    vx, vy, vz = (np.array([vL[..., i], vR[..., i]]) for i in range(3))
+    Bx, By, Bz = (np.array([BL[..., i], BR[..., i]]) for i in range(3))
+    rho_h = am(z1) * lon(z5)
+    P1 = gdiv(am(z5), am(z1))
+    P2 = ((gamma + 1) / (2 * gamma)) * (gdiv(lon(z5), lon(z1))) + ((gamma - 1) / (2 * gamma)) * (gdiv(am(z5), am(z1)))
+    u1, v1, w1 = (gdiv(am(x * z1), am(z1)) for x in (vx, vy, vz))
+    u2, v2, w2 = (gdiv(am(x * z1 ** 2), am(z1 ** 2)) for x in (vx, vy, vz))
+    B1h, B1d, B2h, B2d, B3h, B3d = am(Bx), am(Bx ** 2), am(By), am(By ** 2), am(Bz), am(Bz ** 2)
+    B1B2, B1B3 = am(Bx * By), am(Bx * Bz)
+    ec = np.zeros_like(wp)
+    ec[..., 0] = rho_h * u1
+    ec[..., a + 1] = P1 + rho_h * u1 ** 2 + .5 * (B1d + B2d + B3d) - B1d
+    ec[..., o + 1] = rho_h * u1 * v1 - B1B2
+    ec[..., c + 1] = rho_h * u1 * w1 - B1B3
+    ec[..., 4] = ((gamma / (gamma - 1)) * u1 * P2 + .5 * rho_h * u1 * (u1 ** 2 + v1 ** 2 + w1 ** 2)
+                  + u2 * (B2h ** 2 + B3h ** 2) - B1h * (v2 * B2h + w2 * B3h))
+    ec[..., o + 5] = u2 * B2h - v2 * B1h
+    ec[..., c + 5] = u2 * B3h - w2 * B1h
+    hat = np.stack([rho_h, u1, v1, w1, P1, B1h, B2h, B3h], axis=-1)
+    R = es_right_eigenvectors(ax, hat, gamma)
+    _, _, cax, cf, csl = _fast_slow(rho_h, P1, hat[..., 5:8], B1h, gamma)
+    waves = (u1 + cf, u1 + cax, u1 + csl, u1, u1, u1 - csl, u1 - cax, u1 - cf)
+    roe = np.zeros_like(R)
+    for k, x in enumerate(waves):
+        roe[..., k, k] = x
+    roe = np.abs(roe)
+    mx = np.maximum.reduce([np.abs(x) for x in waves[:4] + waves[5:]])
+    lff = np.zeros_like(R)
+    for k in range(8):
+        lff[..., k, k] = mx
+    e = np.zeros_like(wp)
+    e[..., 0] = (((gamma - np.log(PL * rhoL ** -gamma)) / (gamma - 1) - gdiv(.5 * rhoL * norm2(vL), PL))
+                 - ((gamma - np.log(PR * rhoR ** -gamma)) / (gamma - 1) - gdiv(.5 * rhoR * norm2(vR), PR)))
+    e[..., 4] = gdiv(rhoR, PR) - gdiv(rhoL, PL)
+    e[..., 1:4] = gdiv(vL * rhoL[..., None], PL[..., None]) - gdiv(vR * rhoR[..., None], PR[..., None])
+    e[..., 5:8] = gdiv(BL * rhoL[..., None], PL[..., None]) - gdiv(BR * rhoR[..., None], PR[..., None])
+    e *= -1
+    eps = np.sqrt(np.abs(gdiv(PR - PL, PR + PL)))
+    lam = (1 - eps)[..., None, None] * roe + eps[..., None, None] * lff
+    absA = R @ lam @ R.transpose(0, 2, 1)
+    diss = (absA @ e[..., None])[..., 0]
+    return ec + .5 * diss
 
 
 def riemann(s, ax, lmax, chars, d):

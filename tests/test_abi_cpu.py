@@ -64,8 +64,7 @@ def test_solver_routing():
     for name in ("os", "osher", "osher-solomon", "osher solomon"):
         assert O.solver_of(name) == L.AH_DOTS
     for name in ("solomon", "es", "entropy", "entropy-stable"):
-        with pytest.raises(NotImplementedError):
-            O.solver_of(name)
+        assert O.solver_of(name) == L.AH_ES
 
 
 def test_invalid_values_revert_to_defaults():

@@ -129,6 +129,83 @@ def test_dots_interface_flux_1d(key):
     assert e_or <= 1e-11 and e_ref <= 1e-11
 
 
+@pytest.mark.parametrize("key", DOTS_KEYS)
+def test_es_interface_flux_1d(key):
+    """Entropy-stable interface fluxes (solvers.py:288-391) against the
+    reference's (schemes_1d.npz *_flux_es) and the oracle: the same entries
+    are non-finite (most of them, the reference's averages being
+    half-differences), the finite ones agree to rounding (the reference's
+    8x8 products go through BLAS)."""
+    from astrea_amd.plan import Plan
+    g = load("schemes_1d.npz")
+    phys, sg, bc = key.split("_")
+    gamma = 5 / 3 if phys == "mhd" else 1.4
+    grid = g[key + "_grid"]
+    N = grid.shape[0]
+    _say(f"[gpu] es flux1d {key}")
+    plan = Plan(1, sg, "es", "euler", [dict(n0=N, boundary=bc, dx=1 / 32, gamma=gamma, cfl=.5, t_end=1.)],
+                varmask=0xFF)
+    plan.load([grid])
+    fl = plan.evolve_space(0)
+    plan.close()
+    s = O.Scheme(dimension=1, subgrid=sg, solver="es", boundary=bc, gamma=gamma, dx=1 / 32)
+    d = O.reconstruct(grid, s, 0)
+    with np.errstate(all="ignore"):
+        F_or = O.riemann(s, 0, None, None, d)
+    F_ref = g[key + "_flux_es"]
+    got = fl[0]["flux"]
+    assert np.array_equal(np.isfinite(got), np.isfinite(F_ref))
+    assert np.array_equal(np.isfinite(F_or), np.isfinite(F_ref))
+    fin = np.isfinite(F_ref)
+    big = np.max(np.abs(np.where(fin, F_ref, 0)))
+    scale = np.maximum(np.max(np.abs(np.where(fin, F_ref, 0)), axis=0), 1e-4 * big)
+    e_or = np.max(np.where(fin, np.abs(got - np.where(fin, F_or, 0)) / scale, 0))
+    e_ref = np.max(np.where(fin, np.abs(got - np.where(fin, F_ref, 0)) / scale, 0))
+    _say(f"[gpu]   {int((~fin).sum())} of {fin.size} non-finite in both; finite max scaled error vs oracle "
+         f"{e_or:.2e}, vs reference {e_ref:.2e}")
+    assert e_or <= 1e-11 and e_ref <= 1e-11
+
+
+def test_es_run_fails_where_the_reference_does():
+    """A 1D run with the entropy-stable flux: the state turns non-finite in
+    the oracle after the first step; the device raises the failure flag at
+    the next stage-0 spectral radius (where the reference's LAPACK call would
+    raise) and keeps the last finite state."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    n = 64
+    w = ic.riemann_1d(n, (1., 0., 1.), (.125, 0., .1))
+    q0 = ic.initial_average(torch.from_numpy(w), 1.4, "edge", False).numpy()
+    _say("[gpu] es 1D sod 64 run")
+    plan = Plan(1, "plm", "es", "ssprk(3,3)", [dict(n0=n, boundary="edge", dx=1 / n, gamma=1.4, cfl=.5, t_end=1e9,
+                                                     max_steps=5)], varmask=0xFF)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=5)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=1, subgrid="plm", solver="es", timestep="ssprk(3,3)", boundary="edge", gamma=1.4, cfl=.5,
+                 dx=1 / n)
+    first_bad = None
+    with np.errstate(all="ignore"):
+        for k in range(1, 6):
+            r = O.run(q0, s, 1e9, max_steps=k)
+            if not np.isfinite(r.grid).all():
+                first_bad = k
+                break
+    _say(f"[gpu]   oracle non-finite after step {first_bad}; device steps {st['steps']} failed {st['failed']}")
+    assert first_bad is not None
+    assert st["failed"] and st["steps"] == first_bad - 1
+    assert np.isfinite(out).all()
+    if first_bad > 1:
+        with np.errstate(all="ignore"):
+            ok = O.run(q0, s, 1e9, max_steps=first_bad - 1)
+        assert np.max(rel_l1(out, ok.grid)) <= 1e-12
+    else:
+        assert np.array_equal(out, q0)
+
+
 @pytest.mark.parametrize("dims,subgrid", [(1, "plm"), (1, "ppm"), (2, "plm")])
 def test_dots_trajectory_vs_oracle(dims, subgrid):
     """Osher-Solomon runs (1D Riemann problem 64 cells, 2D quadrants 16^2),
