@@ -3,8 +3,10 @@
 Requests (2D problems of different sizes, schemes and end times) arrive over
 time.  Admitted requests keep their state resident on the device; requests
 of one scheme share one batched plan, each with its own t and dt on the
-device.  Admission is against an HBM budget: a request waits until the bytes
-of the active set plus its own fit.  Whenever the active set of a scheme
+device.  Admission is against an HBM budget: a request is admitted when the
+re-plans it causes stay within it -- the groups re-plan one at a time, each
+new plan built while its old one is alive, and the peak of that sequence
+(computed in Server._replan_peak) must fit.  Whenever the active set of a scheme
 changes (an admission or a completion) its plan is rebuilt and the running
 requests' state planes and time states move over (device-to-device copies),
 so new work joins the batch between chunks of steps instead of waiting for
@@ -117,6 +119,12 @@ class _Group:
             else:
                 self.plan.step(chunk)
 
+    def held(self, cost):
+        """Bytes of the group's current plan: a re-plan builds the new plan
+        while this one is alive (the running requests' planes are copied
+        over), so admission has to leave room for both."""
+        return sum(cost(r) for r in getattr(self, "_planned", []))
+
     def harvest(self, now, keep_results):
         """Finished requests (done or failed) leave the group."""
         if self.plan is None:
@@ -154,10 +162,30 @@ class Server:
     def cost(self, r):
         return r.n * r.n * self.bytes_per_cell
 
+    def _replan_peak(self, extra):
+        """Most bytes the plans hold at once if ``extra`` (request costs per
+        scheme group) is admitted now: the stale groups re-plan one at a time
+        in the loop's order, each new plan built while its old one is alive;
+        afterwards, whenever a request finishes, its group re-plans without
+        it the same way (a group of one just closes), so room is kept for the
+        largest such shrink as well."""
+        keys = list(self.groups) + [k for k in extra if k not in self.groups]
+        cur = {k: (self.groups[k].held(self.cost) if k in self.groups else 0) for k in keys}
+        members = {k: ([self.cost(r) for r in self.groups[k].reqs] if k in self.groups else []) + list(extra.get(k, []))
+                   for k in keys}
+        total = sum(cur.values())
+        peak = total
+        for k in keys:
+            if (k in self.groups and self.groups[k].stale) or k in extra:
+                new = sum(members[k])
+                peak = max(peak, total + new)
+                total += new - cur[k]
+        shrink = max((sum(m) - min(m) for m in members.values() if len(m) > 1), default=0)
+        return max(peak, total + shrink)
+
     def run(self, requests, realtime=True):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
-        used = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         finished = []
@@ -166,9 +194,20 @@ class Server:
             tn = now()
             while pending and (pending[0].arrival <= tn or not realtime):
                 waiting.append(pending.popleft())
-            while waiting and used + self.cost(waiting[0]) <= self.budget:
-                r = waiting.popleft()
-                used += self.cost(r)
+            extra, admitted = {}, []
+            while waiting:
+                r = waiting[0]
+                trial = {k: list(v) for k, v in extra.items()}
+                trial.setdefault(r.subgrid, []).append(self.cost(r))
+                if self._replan_peak(trial) > self.budget:
+                    break
+                extra = trial
+                admitted.append(waiting.popleft())
+            idle = not admitted and not any(g.reqs or g.stale for g in self.groups.values())
+            if waiting and idle and self._replan_peak({waiting[0].subgrid: [self.cost(waiting[0])]}) > self.budget:
+                raise ValueError(f"request {waiting[0].rid} ({self.cost(waiting[0])} B) does not fit the HBM "
+                                 f"budget of {self.budget} B")
+            for r in admitted:
                 r.admitted_at = now()
                 g = self.groups.get(r.subgrid)
                 if g is None:
@@ -186,9 +225,7 @@ class Server:
             for g in self.groups.values():
                 g.advance(self.chunk)
             for g in self.groups.values():
-                for r in g.harvest(now, self.keep_results):
-                    used -= self.cost(r)
-                    finished.append(r)
+                finished.extend(g.harvest(now, self.keep_results))
         for g in self.groups.values():
             if g.plan:
                 g.plan.close()
