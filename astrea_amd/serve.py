@@ -149,8 +149,14 @@ class _Group:
 
 class Server:
     def __init__(self, hbm_budget=32 << 30, precision="fp64", compat="physical", cfl=.4, chunk=16,
-                 device="cuda:0", keep_results=False, bytes_per_cell=None):
+                 device="cuda:0", keep_results=False, bytes_per_cell=None, max_active_cells=None):
         self.budget = hbm_budget
+        # beyond the cells that fill the GPU, admitting more work adds no
+        # throughput and only stretches the running requests (processor
+        # sharing): an optional cap keeps the excess queued in arrival order,
+        # trading median latency for tail latency.  A request larger than the
+        # cap still runs, alone.
+        self.max_active_cells = max_active_cells
         self.precision, self.compat, self.cfl, self.chunk = precision, compat, cfl, chunk
         self.device = device
         self.keep_results = keep_results
@@ -195,14 +201,19 @@ class Server:
             while pending and (pending[0].arrival <= tn or not realtime):
                 waiting.append(pending.popleft())
             extra, admitted = {}, []
+            active_cells = sum(x.n * x.n for g in self.groups.values() for x in g.reqs)
             while waiting:
                 r = waiting[0]
+                if (self.max_active_cells is not None and active_cells > 0
+                        and active_cells + r.n * r.n > self.max_active_cells):
+                    break
                 trial = {k: list(v) for k, v in extra.items()}
                 trial.setdefault(r.subgrid, []).append(self.cost(r))
                 if self._replan_peak(trial) > self.budget:
                     break
                 extra = trial
                 admitted.append(waiting.popleft())
+                active_cells += r.n * r.n
             idle = not admitted and not any(g.reqs or g.stale for g in self.groups.values())
             if waiting and idle and self._replan_peak({waiting[0].subgrid: [self.cost(waiting[0])]}) > self.budget:
                 raise ValueError(f"request {waiting[0].rid} ({self.cost(waiting[0])} B) does not fit the HBM "

@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=3.0,
                     help="serve: arrivals per second (3.0/s loads the plain FP64 loop to ~83%%: the stream "
                          "averages 2.4e8 zone-updates per request at ~8.6e8 ZU/s)")
+    ap.add_argument("--serve-cap", type=int, default=3 << 20,
+                    help="serve: active-cell cap of the extra capped Poisson run (three 1024^2 requests)")
     return ap.parse_args()
 
 
@@ -188,8 +190,12 @@ def bench_serve(args):
         ref = {r.rid: r for r in plain.pop("done")}
         out = {"plain_fp64": plain}
         _progress(f"[serve] {mode} plain fp64: {plain['zu_per_s']:.3e} ZU/s, p50 {plain['p50']:.2f} s")
-        for prec in ("fp64", "split"):
-            r = serve.Server(precision=prec, keep_results=True).run(requests(realtime), realtime=realtime)
+        runs = [("fp64", "fp64", None), ("split", "split", None)]
+        if realtime:
+            runs.append(("fp64_capped", "fp64", args.serve_cap))
+        for label, prec, cap in runs:
+            r = serve.Server(precision=prec, keep_results=True, max_active_cells=cap).run(
+                requests(realtime), realtime=realtime)
             done = r.pop("done")
             errs, mism = [], 0
             for x in done:
@@ -201,9 +207,9 @@ def bench_serve(args):
             r["l1_vs_plain"] = {"max": max(errs) if errs else None,
                                 "median": float(np.median(errs)) if errs else None, "compared": len(errs),
                                 "failure_mismatch": mism}
-            out[f"served_{prec}"] = r
-            _progress(f"[serve] {mode} served {prec}: {r['zu_per_s']:.3e} ZU/s, p50 {r['p50']:.2f} s, "
-                      f"L1 max {r['l1_vs_plain']['max']}")
+            out[f"served_{label}"] = r
+            _progress(f"[serve] {mode} served {label}: {r['zu_per_s']:.3e} ZU/s, p50 {r['p50']:.2f} s, "
+                      f"p99 {r['p99']:.2f} s, L1 max {r['l1_vs_plain']['max']}")
         modes[mode] = out
     sat = modes["saturated"]
     ok = [p for p in ("served_split", "served_fp64")
@@ -223,6 +229,8 @@ def bench_serve(args):
         "speedup_vs_plain": value / sat["plain_fp64"]["zu_per_s"],
         "p50_s": poi[best]["p50"], "p99_s": poi[best]["p99"],
         "plain_p50_s": poi["plain_fp64"]["p50"], "plain_p99_s": poi["plain_fp64"]["p99"],
+        "capped": {"max_active_cells": args.serve_cap, "p50_s": poi["served_fp64_capped"]["p50"],
+                   "p99_s": poi["served_fp64_capped"]["p99"], "zu_per_s": poi["served_fp64_capped"]["zu_per_s"]},
     }
 
 

@@ -140,3 +140,22 @@ def test_request_larger_than_the_budget_is_refused(fake_gpu):
     srv = serve.Server(hbm_budget=100, chunk=2, device="cpu", bytes_per_cell=1)
     with pytest.raises(ValueError, match="does not fit"):
         srv.run(_requests([(16, "plm", .2, 0.0)]), realtime=False)
+
+
+def test_active_cell_cap_bounds_the_batch(fake_gpu, monkeypatch):
+    cap = 32 * 32 + 16 * 16
+    seen = []
+    advance = serve._Group.advance
+
+    def watch(self, chunk, use_graph=True):
+        seen.append(sum(r.n * r.n for g in srv.groups.values() for r in g.reqs))
+        return advance(self, chunk, use_graph)
+
+    monkeypatch.setattr(serve._Group, "advance", watch)
+    srv = serve.Server(hbm_budget=1 << 40, chunk=2, device="cpu", keep_results=True, bytes_per_cell=1,
+                       max_active_cells=cap)
+    out = srv.run(_requests(SPEC), realtime=False)
+    assert out["requests"] == len(SPEC)
+    assert seen and max(seen) <= cap
+    for r in out["done"]:
+        assert torch.all(r.result == r.status["steps"])
