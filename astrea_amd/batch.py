@@ -17,7 +17,10 @@ from .plan import Plan
 def run(plan: Plan, chunk=16, use_graph=True, compact_every=1, max_chunks=10 ** 7):
     """Advance every problem of ``plan`` to its end.  Returns (status list,
     seconds, number of chunks)."""
-    stream = torch.cuda.Stream(device=plan.device) if use_graph else torch.cuda.current_stream(plan.device)
+    caller = torch.cuda.current_stream(plan.device)
+    stream = torch.cuda.Stream(device=plan.device) if use_graph else caller
+    # the plan's state was loaded on the caller's stream
+    stream.wait_stream(caller)
     t0 = time.perf_counter()
     n = 0
     with torch.cuda.stream(stream):

@@ -24,10 +24,11 @@ inline void set2d(KernelSet* ks) {
     constexpr int NVB = VS<MASK>::NV * (int)sizeof(R);
     constexpr int RTS0 = 32, RTT0 = 8, RTS1 = 64, RTT1 = 4;
     // flux tile: (FTT + 2) x (FTS + 1) face solves and (FTT + 2) x (FTS + 2)
-    // radii per tile each fit one pass of 256 threads, at two workgroups per CU
-    // of LDS (hydro 16 x 12: 65 KB; MHD 16 x 8: 71 KB); measured on the 1024^2
-    // shear layer against 32 x 4, 24 x 8 and 40 x 4
-    constexpr int FTS = 16, FTT = NVB > 32 ? 8 : 12, FNT = 256;
+    // radii per tile each fit one pass of 256 threads; hydro 16 x 10 takes
+    // 50.5 KB of LDS (three workgroups per CU), MHD 16 x 8 64 KB (two, as its
+    // registers allow); measured on the 1024^2 shear layer against 16 x 12,
+    // 16 x 8, 15 x 10, 15 x 14, 31 x 4, 31 x 6, 32 x 4, 24 x 8 and 40 x 4
+    constexpr int FTS = 16, FTT = NVB > 32 ? 8 : 10, FNT = 256;
     ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
         launch_sweep(recon2d_kernel<R, MASK, RECON, 0, RTS0, RTT0, SPLIT>, a, nb, s);
     };

@@ -738,7 +738,13 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     __shared__ R fa_s[NV][NTA][NKF];
     __shared__ R rad_s[NTA][NK];
     __shared__ R bn_s[SOLVER == HLLD ? NTA : 1][SOLVER == HLLD ? NKF : 1];
-    __shared__ R F_s[NV][TT][NKF];
+    // When every thread owns at most one interface of stage D, the corrected
+    // fluxes wait in registers across a barrier and then take the place of
+    // qm_s (dead by then), which saves the F window's LDS: the 16x10 hydro
+    // tile then needs 50.5 KB and three workgroups fit a CU instead of two.
+    constexpr bool FALIAS = TT * NKF <= NTHR;
+    __shared__ R F_own[FALIAS ? 1 : NV][FALIAS ? 1 : TT][FALIAS ? 1 : NKF];
+    R (*F_s)[TT][NKF] = reinterpret_cast<R (*)[TT][NKF]>(FALIAS ? &qm_s[0][0][0] : &F_own[0][0][0]);
     __shared__ double red[NTHR / 64];
 
     const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
@@ -747,8 +753,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     const long long P = a.plane;
     // w-/w+ window of a tile, interfaces [s0, s0+TS+1] x lines [t0-2, t0+TT+2),
     // fetched into registers one tile ahead: the loads of the next tile are in
-    // flight while this tile computes (the LDS windows limit residency to two
-    // workgroups per CU, so the registers are free)
+    // flight while this tile computes (residency is bounded by the LDS windows
+    // before the registers, so the prefetch costs no occupancy)
     constexpr int NA = (NT * NK + NTHR - 1) / NTHR;
     R pfm[NA][NV], pfp[NA][NV];
     auto fetch = [&](int ti) {
@@ -823,6 +829,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     // Tiles whose lines [t0-2, t0+TT+2) and Jacobian positions [s0, s0+TS+1]
     // stay inside the domain (and every periodic tile) skip the boundary map.
     const bool clampT = bc == EDGE && !(t0 >= 2 && t0 + TT + 2 <= nt && s0 >= 1 && s0 + TS + 1 <= ns);
+    R Freg[NV];
     auto stages = [&](auto clamp_tag) {
     constexpr bool CL = decltype(clamp_tag)::value;
     auto et = [&](int p) {
@@ -966,7 +973,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
             if (V::live(c)) {
                 const int v = V::slot(c);
                 const R F = Fc.c[c] - R(K24) * ((fa_s[v][iap][ik] - fa_s[v][ia0][ik]) - (fa_s[v][ia0][ik] - fa_s[v][iam][ik]));
-                F_s[v][itt][ik] = F;
+                if constexpr (FALIAS) Freg[v] = F;
+                else F_s[v][itt][ik] = F;
                 if (a.flux_out && t < nt && k <= ns && (k < s0 + TS || k == ns))
                     a.flux_out[v * a.flux_plane + (long long)k * nt + t] = double(F);
             }
@@ -975,6 +983,15 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     if (clampT) stages(std::true_type{});
     else stages(std::false_type{});
     __syncthreads();
+    if constexpr (FALIAS) {
+        // interface i = threadIdx.x of stage D (one per thread at most)
+        const int i = threadIdx.x;
+        if (i < TT * NKF) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) F_s[v][i / NKF][i % NKF] = Freg[v];
+        }
+        __syncthreads();
+    }
     // E: flux difference; first sweep writes D, second writes L = -(D0 + D1)
     const R dxR = R(pd.dx);
     for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {

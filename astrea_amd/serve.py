@@ -80,6 +80,8 @@ class _Group:
 
     def rebuild(self):
         """New plan over self.reqs; running requests keep state and time."""
+        # new requests' initial states were written on the caller's stream
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(self.stream):
             old, old_reqs = self.plan, getattr(self, "_planned", [])
             if not self.reqs:
@@ -138,7 +140,9 @@ class _Group:
                 r.status = s
                 if keep_results:
                     o = self.plan.offsets[i]
-                    r.result = self.plan.state[:, o:o + r.n * r.n].clone()
+                    with torch.cuda.stream(self.stream):
+                        r.result = self.plan.state[:, o:o + r.n * r.n].clone()
+                    self.stream.synchronize()  # the caller reads it on its own stream
                 done.append(r)
         if done:
             ids = {r.rid for r in done}

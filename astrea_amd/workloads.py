@@ -89,6 +89,7 @@ def autotune_blocks(grids, probs, precision, candidates=(512, 1024, 2048, 4096),
         plan.load(grids)
         plan.reset()
         s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             plan.step_graph(16, 1)
             s.synchronize()

@@ -376,9 +376,10 @@ def main():
             plan.set_blocks(args.blocks)
         plan.load([q0])
         plan.reset()
-        plan.step(args.warmup)
         stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(stream):
+            plan.step(args.warmup)
             plan.step_graph(chunk, 0)
             stream.synchronize()
             start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -401,6 +402,20 @@ def main():
         assert st["steps"] == args.warmup + args.steps + prof_steps and not st["failed"], st
         results[path] = dict(ms=ms, wall=wall, prof=prof, state=plan.state.clone(), plan=plan,
                              eager_ms_per_step=e0.elapsed_time(e1) / prof_steps)
+    # integrity: the graph-replayed trajectory equals eager stepping bit for bit
+    for path, r in results.items():
+        chk = Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], flattener="astrea", precision=path, varmask=0x17,
+                   compat="physical")
+        if args.blocks:
+            chk.set_blocks(args.blocks)
+        chk.load([q0])
+        chk.reset()
+        chk.step(prob["max_steps"])
+        torch.cuda.synchronize()
+        r["graph_matches_eager"] = bool(torch.equal(chk.state, r["state"]))
+        chk.close()
+        if not r["graph_matches_eager"]:
+            _progress(f"[shear] {path}: graph-replayed state differs from eager stepping")
     ref = results.get("fp64")
     l1 = {}
     for path, r in results.items():
@@ -453,6 +468,7 @@ def main():
         "cpu_baseline": cpu,
         "paths": {p: {"zu_per_s": zu / (v["ms"] / 1e3), "ms_per_step": v["ms"] / args.steps,
                       "eager_ms_per_step": v["eager_ms_per_step"], "l1_vs_fp64": l1.get(p, 0.0),
+                      "graph_matches_eager": v["graph_matches_eager"],
                       "kernels_avg_ms": {k: kv[0] / kv[1] for k, kv in v["prof"].items()}}
                   for p, v in results.items()},
         "step_hbm_achieved_GBps": b_alg_zu * value / 1e9,

@@ -307,6 +307,59 @@ def test_conservation_periodic_2d(subgrid, solver):
     assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.maximum(1, np.abs(tot0)))
 
 
+@pytest.mark.parametrize("n,subgrid,bc", [(17, "ppm", "wrap"), (17, "plm", "edge"), (33, "ppm", "edge"),
+                                          (25, "weno5", "wrap"), (13, "ppm", "edge"), (65, "plm", "wrap")])
+def test_ragged_tiles_2d(n, subgrid, bc):
+    """Grids whose last reconstruction and flux tiles hold a single column
+    or row (16k+1, 8k+1, 4k+1 cells; tiles of 32x8/64x4 cells and 16x12
+    interfaces) or a few: device FP64 equals the oracle after 6 SSPRK(3,3)
+    steps."""
+    rng = np.random.default_rng(n)
+    c = (np.arange(n) + .5) / n
+    x, y = np.meshgrid(c, c, indexing="ij")
+    w = np.zeros((n, n, 8))
+    w[..., 0] = 1 + .3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) + .02 * rng.uniform(-1, 1, (n, n))
+    w[..., 1] = .2 * np.cos(2 * np.pi * y)
+    w[..., 2] = .1 * np.sin(2 * np.pi * x) + .01 * rng.uniform(-1, 1, (n, n))
+    w[..., 4] = 1.5 + .1 * (x > .5)
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver="hllc", boundary=bc, dx=1 / n, cfl=.4)
+    q0 = O.initial_average(w, s)
+    m = dict(dimension=2, subgrid=subgrid, solver="hllc", timestep="ssprk(3,3)", boundary=bc, gamma=1.4,
+             cfl=.4, dx=1 / n, t_end=10., checkpoints=1, max_steps=6, magnetic_2d=False)
+    _say(f"[gpu] ragged tiles {n}^2 {subgrid} {bc}")
+    out, st = gpu_run(q0, m)
+    ref = oracle_run(q0, m)
+    assert st["steps"] == ref.steps == 6 and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
+def test_mhd_ragged_tiles():
+    """Orszag-Tang on 17^2 (single-column last tiles in every kernel of the
+    constrained-transport path), WENO5 + HLLD, 4 steps: device vs oracle."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    n = 17
+    q0 = ic.initial_average(torch.from_numpy(ic.orszag_tang(n)), 5 / 3, "wrap", True).numpy()
+    _say(f"[gpu] mhd ragged tiles {n}^2")
+    plan = Plan(2, "weno5", "hlld", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=5 / 3, cfl=.4,
+                                                      t_end=1e9, max_steps=4)], varmask=0x77, magnetic_2d=True)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=4)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid="weno5", solver="hlld", boundary="wrap", gamma=5 / 3, cfl=.4, dx=1 / n,
+                 magnetic_2d=True)
+    ref = O.run(q0, s, 1e9, max_steps=4)
+    assert st["steps"] == ref.steps == 4 and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
 def _shear(n, seed=20261202):
     from astrea_amd import ic
     import torch
@@ -456,6 +509,35 @@ def test_batch_equals_alone(use_graph):
         assert s1["steps"] == st[i]["steps"] and s1["failed"] == st[i]["failed"], (i, s1, st[i])
         assert np.array_equal(o1, outs[i]), i
     _say(f"[gpu]   {nch} chunks, failed={sum(s['failed'] for s in st)}")
+
+
+def test_graph_chunks_after_eager_steps_equal_eager():
+    """Eager steps on the caller's stream followed at once by graph-replayed
+    chunks on batch.run's own stream (which must wait for the caller's
+    work) give the same state, bit for bit, as stepping eagerly throughout:
+    2D shear layer 512^2, PPM + flattener, 6 + 12 steps."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import batch, ic
+    import torch
+    n = 512
+    q0 = ic.to_device(ic.shear_layer(n, 20261202), 1.4, "wrap", True)
+    prob = dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9, max_steps=18)
+    _say(f"[gpu] graph chunks after eager steps, shear {n}^2")
+    out = []
+    for mixed in (False, True):
+        plan = Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], flattener="astrea", varmask=0x17, compat="physical")
+        plan.load([q0])
+        plan.reset()
+        if mixed:
+            plan.step(6)
+            st, _, _ = batch.run(plan, chunk=6, use_graph=True)
+        else:
+            st = plan.run(chunk=18)
+        out.append((plan.state.clone(), st[0]["steps"], st[0]["t"]))
+        plan.close()
+    torch.cuda.synchronize()
+    assert out[0][1] == out[1][1] == 18 and out[0][2] == out[1][2]
+    assert torch.equal(out[0][0], out[1][0])
 
 
 @pytest.mark.parametrize("compat", ["astrea", "physical"])

@@ -83,6 +83,9 @@ class _Stream:
     def synchronize(self):
         pass
 
+    def wait_stream(self, other):
+        pass
+
 
 @pytest.fixture
 def fake_gpu(monkeypatch):
@@ -90,6 +93,7 @@ def fake_gpu(monkeypatch):
     monkeypatch.setattr(serve, "Plan", FakePlan)
     monkeypatch.setattr(torch.cuda, "Stream", _Stream)
     monkeypatch.setattr(torch.cuda, "stream", lambda s: contextlib.nullcontext())
+    monkeypatch.setattr(torch.cuda, "current_stream", lambda *a, **k: _Stream())
 
 
 def _requests(spec):
