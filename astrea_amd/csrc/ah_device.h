@@ -80,6 +80,17 @@ template <class R>
 __device__ inline R nmax(R a, R b) { return (a > b || a != a) ? a : b; }  // np.maximum
 template <class R>
 __device__ inline R nsign(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : x); }
+// Hardware min/max (v_min_f64 / v_max_f64).  On numbers they equal
+// np.minimum / np.maximum except in the sign of a zero result; given a NaN
+// they return the other operand where numpy returns NaN.  Used only where the
+// operands hold no negative zero (absolute values) or the sign of a zero
+// result cannot change what is computed from it, and where a NaN operand can
+// only come from a NaN cell of the stencil, whose own interface states are
+// NaN through the arithmetic, so that the step fails as numpy's does.
+__device__ inline double hmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ inline float hmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ inline double hmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ inline float hmax(float a, float b) { return __builtin_fmaxf(a, b); }
 template <class R>
 __device__ inline R sq(R x) { return x * x; }
 
@@ -276,13 +287,16 @@ __device__ inline void mc_limit(R wS, R wm1, R wp1, R wm2, R wp2, R fL, R fR, R 
     wR = fR;
     const R d2w = R(6) * (fL - R(2) * wS + fR);
     const bool cell_extrema = (dwm * dwp <= R(0)) || ((wS - wm2) * (wp2 - wS) <= R(0));
-    const R lim_curv = nsign(d2c) * nmin(nmin(ah_fabs(d2w), C * ah_fabs(d2c)), nmin(C * ah_fabs(d2p), C * ah_fabs(d2m)));
+    // min/max over absolute values: no negative zeros (hmin/hmax exact)
+    const R lim_curv = nsign(d2c) * hmin(hmin(ah_fabs(d2w), C * ah_fabs(d2c)), hmin(C * ah_fabs(d2p), C * ah_fabs(d2m)));
     const R d2w_lim = cell_extrema ? lim_curv : R(0);
-    const bool sens = ah_fabs(d2w) > R(1e-12) * nmax(ah_fabs(wS), nmax(nmax(ah_fabs(wm1), ah_fabs(wp1)), nmax(ah_fabs(wm2), ah_fabs(wp2))));
+    const bool sens = ah_fabs(d2w) > R(1e-12) * hmax(ah_fabs(wS), hmax(hmax(ah_fabs(wm1), ah_fabs(wp1)), hmax(ah_fabs(wm2), ah_fabs(wp2))));
     const R rho = sens ? gdiv(d2w_lim, d2w) : R(0);
-    const R d3min = nmin(nmin(d3m1, d30), nmin(d3m2, d3p2));
-    const R d3max = nmax(nmax(d3m1, d30), nmax(d3m2, d3p2));
-    const bool roundoff = (rho < R(1.0 - 1e-12)) || (R(0.1) * nmax(ah_fabs(d3max), ah_fabs(d3min)) <= (d3max - d3min));
+    // d3min/d3max enter only through |.| and d3max - d3min, which compare
+    // the same against 0.1 max(|d3max|, |d3min|) whatever the zeros' signs
+    const R d3min = hmin(hmin(d3m1, d30), hmin(d3m2, d3p2));
+    const R d3max = hmax(hmax(d3m1, d30), hmax(d3m2, d3p2));
+    const bool roundoff = (rho < R(1.0 - 1e-12)) || (R(0.1) * hmax(ah_fabs(d3max), ah_fabs(d3min)) <= (d3max - d3min));
     if ((dwm * dwp < R(0)) && roundoff) {
         wL = wS - rho * dwm;
         wR = wS + rho * dwp;

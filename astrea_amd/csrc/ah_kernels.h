@@ -128,11 +128,22 @@ __device__ inline V8<double> ho_cell_prim(const V8<double>& q, const V8<double>&
     return r;
 }
 
+// The flattener's chi~ (ppm.py:111-124) at position mm, from the pressure at
+// e(mm)-2..e(mm)+2 and the normal velocity at e(mm)-1, e(mm)+1.
+template <class R, int AX, class Acc>
+__device__ inline R flat_chi_at(const Acc& W, int flattener, int mm) {
+    const int e2 = W.e(mm);
+    const auto p2 = W(e2 - 2), p1 = W(e2 - 1), q1 = W(e2 + 1), q2 = W(e2 + 2);
+    return flat_chi<R>(flattener, p2.c[4], p1.c[4], q1.c[4], q2.c[4], p1.c[AX + 1], q1.c[AX + 1]);
+}
+
 // One cell's reconstruction from a window accessor W(m) that returns the
 // primitive state at sweep position m (already boundary mapped).  `ce` is the
-// evaluation position of the cell.
-template <class R, int MASK, int RECON, int AX, class Acc>
-__device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flattener, V8<R>& wL, V8<R>& wR) {
+// evaluation position of the cell; chi_at(mm) returns flat_chi_at(W, mm)
+// (computed, or read from a table of the row).
+template <class R, int MASK, int RECON, int AX, class Acc, class ChiF>
+__device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flattener, V8<R>& wL, V8<R>& wR,
+                                  const ChiF& chi_at) {
     using V = VS<MASK>;
     wL = zero8<R, MASK>();
     wR = zero8<R, MASK>();
@@ -155,12 +166,6 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
         // padded intermediate at position m is evaluated at W.e(m).
         R etaL = R(1), etaR = R(1);
         if (flattener != FLAT_OFF) {
-            // chi~ (ppm.py:111-124) at me, from p(me-2..me+2) and u(me-1), u(me+1)
-            auto chi_at = [&](int mm) {
-                const int e2 = W.e(mm);
-                const V8<R> p2 = W(e2 - 2), p1 = W(e2 - 1), q1 = W(e2 + 1), q2 = W(e2 + 2);
-                return flat_chi<R>(flattener, p2.c[4], p1.c[4], q1.c[4], q2.c[4], p1.c[AX + 1], q1.c[AX + 1]);
-            };
             // eta(me) = min(chi(me), chi(me -/+ 1)) by sign of the pressure jump (ppm.py:127-131)
             auto eta_at = [&](int m) {
                 const int me = W.e(m);
@@ -219,6 +224,12 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
                          wR.c[c]);
         }
     }
+}
+
+template <class R, int MASK, int RECON, int AX, class Acc>
+__device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flattener, V8<R>& wL, V8<R>& wR) {
+    recon_cell<R, MASK, RECON, AX>(W, ce, limiter, flattener, wL, wR,
+                                   [&](int mm) { return flat_chi_at<R, AX>(W, flattener, mm); });
 }
 
 template <int MASK, bool SPLIT>
@@ -585,6 +596,9 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
     __shared__ double qd[NV][NQT][NQS];
     __shared__ double cq[HO ? NV : 1][HO ? NQT : 1][HO ? NQS : 1];
     __shared__ R ws[NV][TT][NWS];
+    // PPM flattener: chi~ of each own row at positions [s0-2, s0+TS]
+    constexpr int NCHI = TS + 3;
+    __shared__ R chi_s[RECON == PPM ? TT : 1][RECON == PPM ? NCHI : 1];
 
     const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
@@ -665,13 +679,29 @@ __global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
     const bool clampS = bc == EDGE && !(s0 >= HS && s0 + TS + HS <= ns);
     auto cells = [&](auto clamp_tag) {
     constexpr bool CL = decltype(clamp_tag)::value;
+    if constexpr (RECON == PPM) {
+        // each cell's face etas read chi~ at me-1, me, me+1 (me = e(ce-1),
+        // e(ce)): one evaluation per position instead of up to four per cell
+        if (a.flattener != FLAT_OFF) {
+            for (int i = threadIdx.x; i < TT * NCHI; i += blockDim.x) {
+                const int it = i / NCHI, j = i % NCHI;
+                const RowAcc<R, MASK, TT, NWS, CL> W{ws, it, bws, ns};
+                chi_s[it][j] = flat_chi_at<R, AX>(W, a.flattener, s0 - 2 + j);
+            }
+            __syncthreads();
+        }
+    }
     for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
         const int it = i / TS, is = i % TS;
         const int s = s0 + is, t = t0 + it;
         if (s >= ns || t >= nt) continue;
         const RowAcc<R, MASK, TT, NWS, CL> W{ws, it, bws, ns};
         V8<R> L, Rr;
-        recon_cell<R, MASK, RECON, AX>(W, s, a.limiter, a.flattener, L, Rr);
+        if constexpr (RECON == PPM)
+            recon_cell<R, MASK, RECON, AX>(W, s, a.limiter, a.flattener, L, Rr,
+                                           [&](int mm) { return chi_s[it][mm - (s0 - 2)]; });
+        else
+            recon_cell<R, MASK, RECON, AX>(W, s, a.limiter, a.flattener, L, Rr);
         const long long o = fo + (long long)t * ns + s;
 #pragma unroll
         for (int c = 0; c < 8; ++c)
