@@ -344,6 +344,9 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     __shared__ R rad[NR];
     __shared__ R fl[NV][NF];
     __shared__ double red[4];
+    // PPM flattener: chi~ at positions [s0-3, s0+TS+2]
+    constexpr int NCHI = TS + 6;
+    __shared__ R chi_s[RECON == PPM ? NCHI : 1];
 
     const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
@@ -432,9 +435,22 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         auto recon_cells = [&](auto clamp_tag) {
             constexpr bool CL = decltype(clamp_tag)::value;
             const LineAcc<R, MASK, NW, CL> Wc{ws, bw, N};
+            if constexpr (RECON == PPM) {
+                // the cells' face etas read chi~ at me-1, me, me+1 with
+                // me = e(ce-1), e(ce): one evaluation per position
+                if (a.flattener != FLAT_OFF) {
+                    for (int i = threadIdx.x; i < NCHI; i += blockDim.x)
+                        chi_s[i] = flat_chi_at<R, 0>(Wc, a.flattener, s0 - 3 + i);
+                    __syncthreads();
+                }
+            }
             for (int i = threadIdx.x; i < NC; i += blockDim.x) {
                 V8<R> L, Rr;
-                recon_cell<R, MASK, RECON, 0>(Wc, Wc.e(bcl + i), a.limiter, a.flattener, L, Rr);
+                if constexpr (RECON == PPM)
+                    recon_cell<R, MASK, RECON, 0>(Wc, Wc.e(bcl + i), a.limiter, a.flattener, L, Rr,
+                                                  [&](int mm) { return chi_s[mm - (s0 - 3)]; });
+                else
+                    recon_cell<R, MASK, RECON, 0>(Wc, Wc.e(bcl + i), a.limiter, a.flattener, L, Rr);
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
                     if (V::live(c)) {
