@@ -110,6 +110,27 @@ struct ah_plan {
 };
 
 namespace {
+// Tile lists in XCD order.  Workgroup b runs on XCD b % 8 (workgroups are
+// dispatched round-robin over the eight XCDs) and walks list positions b,
+// b + grid, ...; with a grid that is a multiple of 8, the positions equal to
+// x modulo 8 are therefore XCD x's.  Giving each XCD a contiguous band of the
+// row-major tile order keeps the halo rows that neighbouring tiles share in
+// one L2 instead of fetching them into several.
+void xcd_order(std::vector<Tile>& v) {
+    constexpr size_t NX = 8;
+    const size_t n = v.size();
+    if (n < 2 * NX) return;
+    const size_t band = (n + NX - 1) / NX;
+    std::vector<Tile> o;
+    o.reserve(n);
+    for (size_t j = 0; j < band; ++j)
+        for (size_t x = 0; x < NX; ++x) {
+            const size_t s = x * band + j;
+            if (s < n) o.push_back(v[s]);
+        }
+    v.swap(o);
+}
+
 struct ProfScope {
     ah_plan* p;
     hipStream_t s;
@@ -229,6 +250,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
         delete p;
         return AH_ERR_INVALID;
     }
+    for (int l : {0, 1, 2, 3, 4, 6, 7}) xcd_order(p->tl[l].full);
     bool up = hipMalloc((void**)&p->d_pd, nprob * sizeof(ProbDesc)) == hipSuccess &&
               hipMemcpy(p->d_pd, pd.data(), nprob * sizeof(ProbDesc), hipMemcpyHostToDevice) == hipSuccess &&
               hipMalloc((void**)&p->d_ps, nprob * sizeof(ProbState)) == hipSuccess &&
