@@ -28,7 +28,7 @@ inline void set2d(KernelSet* ks) {
     // 50.5 KB of LDS (three workgroups per CU), MHD 16 x 8 64 KB (two, as its
     // registers allow); measured on the 1024^2 shear layer against 16 x 12,
     // 16 x 8, 15 x 10, 15 x 14, 31 x 4, 31 x 6, 32 x 4, 24 x 8 and 40 x 4
-    constexpr int FTS = 16, FTT = NVB > 32 ? 8 : 10, FNT = 256;
+    constexpr int FTS0 = 16, FTT0 = NVB > 32 ? 8 : 10, FTS1 = 16, FTT1 = NVB > 32 ? 8 : 10, FNT = 256;
     ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
         launch_sweep(recon2d_kernel<R, MASK, RECON, 0, RTS0, RTT0, SPLIT>, a, nb, s);
     };
@@ -36,17 +36,19 @@ inline void set2d(KernelSet* ks) {
         launch_sweep(recon2d_kernel<R, MASK, RECON, 1, RTS1, RTT1, SPLIT>, a, nb, s);
     };
     ks->flux[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 0, FTS, FTT, SPLIT, FNT>, a, nb, s, FNT);
+        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 0, FTS0, FTT0, SPLIT, FNT>, a, nb, s, FNT);
     };
     ks->flux[1] = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 1, FTS, FTT, SPLIT, FNT>, a, nb, s, FNT);
+        launch_sweep(flux2d_kernel<R, MASK, SOLVER, CONV, 1, FTS1, FTT1, SPLIT, FNT>, a, nb, s, FNT);
     };
     ks->rts[0] = RTS0;
     ks->rtt[0] = RTT0;
     ks->rts[1] = RTS1;
     ks->rtt[1] = RTT1;
-    ks->fts[0] = ks->fts[1] = FTS;
-    ks->ftt[0] = ks->ftt[1] = FTT;
+    ks->fts[0] = FTS0;
+    ks->ftt[0] = FTT0;
+    ks->fts[1] = FTS1;
+    ks->ftt[1] = FTT1;
     ks->real_bytes = sizeof(R);
     if constexpr (VS<MASK>::MHD && !SPLIT) {
         constexpr int TTS = 64, TTT = 8;

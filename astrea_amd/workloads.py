@@ -79,7 +79,7 @@ def run_ensemble_plain(grids, probs, sample=None, chunk=64, compat="physical"):
     return dict(seconds=secs, zu=zu, zu_per_s=zu / secs, status=st)
 
 
-def autotune_blocks(grids, probs, precision, candidates=(512, 1024, 2048, 4096), steps=64, compat="physical"):
+def autotune_blocks(grids, probs, precision, candidates=(1024, 2048, 4096, 8192), steps=64, compat="physical"):
     """Pick the persistent grid cap on the problem set itself: time a
     fixed number of steps for each candidate (state restored between)."""
     best, times = None, {}
