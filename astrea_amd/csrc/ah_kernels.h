@@ -782,7 +782,11 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     __shared__ R fm_s[NV][NTA][NKF];
     __shared__ R fp_s[NV][NTA][NKF];
     __shared__ R fa_s[NV][NTA][NKF];
-    __shared__ R rad_s[NTA][NK];
+    // only LLF reads the radii of the halo positions (its lambda); the other
+    // solvers need the radius of the tile's own interfaces (dt reduction and
+    // the non-finite check), which stage B then evaluates alone
+    constexpr bool LAM = SOLVER == LLF;
+    __shared__ R rad_s[LAM ? NTA : 1][LAM ? NK : 1];
     __shared__ R bn_s[SOLVER == HLLD ? NTA : 1][SOLVER == HLLD ? NKF : 1];
     // When every thread owns at most one interface of stage D, the corrected
     // fluxes wait in registers across a barrier and then take the place of
@@ -883,11 +887,13 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
         else return p;
     };
     // B: spectral radius at padded Jacobian positions j in [s0, s0+TS+1],
-    // lines [t0-1, t0+TT+1) (evaluated on the boundary-mapped line)
+    // lines [t0-1, t0+TT+1) (evaluated on the boundary-mapped line); without
+    // LLF only at the own positions j in [s0+1, s0+TS], lines [t0, t0+TT)
     double lmax = 0.0;
     bool bad = false;
-    for (int i = threadIdx.x; i < NTA * NK; i += blockDim.x) {
-        const int ia = i / NK, ik = i % NK;
+    constexpr int NB = LAM ? NTA * NK : TT * TS;
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) {
+        const int ia = LAM ? i / NK : 1 + i / TS, ik = LAM ? i % NK : 1 + i % TS;
         const int t = bta + ia, te = et(t), it = te - bt;
         const int j = s0 + ik;
         V8<R> st;
@@ -904,7 +910,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
         R r;
         if constexpr (SPLIT) r = radius<MASK, AX, true>(st, pd.gamma);
         else r = spectral_radius<R, MASK, AX>(st, gR);
-        rad_s[ia][ik] = r;
+        if constexpr (LAM) rad_s[ia][ik] = r;
         if (t >= t0 && t < t0 + TT && t < nt && j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= ns) {
             const double rd = double(r);
             if (!(rd <= 1.7976931348623157e308)) bad = true;
@@ -966,7 +972,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
             Bn = reinterpret_cast<const R*>(a.bn)[sidx(bcmap(k, ns, bc), gt)];
             bn_s[ia][ik] = Bn;
         }
-        const R lam = nmax(rad_s[ia][ik], rad_s[ia][ik + 1]);
+        const R lam = LAM ? nmax(rad_s[ia][ik], rad_s[ia][ik + 1]) : R(0);
         const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0, SPLIT);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -1009,7 +1015,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
                 f.fm.c[c] = corrA(fm_s, v);
                 f.fp.c[c] = corrA(fp_s, v);
             }
-        const R lam = nmax(rad_s[ia0][ik], rad_s[ia0][ik + 1]);
+        const R lam = LAM ? nmax(rad_s[ia0][ik], rad_s[ia0][ik + 1]) : R(0);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = bn_s[ia0][ik];
         const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0, SPLIT);
