@@ -522,19 +522,19 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             const int nc = blocks_for(p, p->tl[5]);
             p->ks.trecon[0](m, blocks_for(p, p->tl[6]), s);
             p->ks.trecon[1](m, blocks_for(p, p->tl[7]), s);
-            p->ks.corner(m, nc, s);
+            p->ks.corner(m, blocks_for(p, p->tl[7]), s);
             p->ks.magL(m, nc, s);
         }
         if (st == 0) {
             ProfScope ps_(p, AH_K_DT, s);
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
-        if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[5]), s);
+        if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[7]), s);
         launch_combine(p, S.op, s);
         if (mhd) {
             double* K = p->regs[target_reg(S.op)];
             m.K = K;
-            p->ks.refine(m, blocks_for(p, p->tl[5]), s);
+            p->ks.refine(m, blocks_for(p, p->tl[7]), s);
             p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
         }
     }

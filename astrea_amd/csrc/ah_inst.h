@@ -61,16 +61,16 @@ inline void set2d(KernelSet* ks) {
             if (nb > 0) hipLaunchKernelGGL((trecon_kernel<R, MASK, 1, TTS, TTT>), dim3(nb), dim3(256), 0, s, a);
         };
         ks->corner = [](const MhdArgs& a, int nb, hipStream_t s) {
-            if (nb > 0) hipLaunchKernelGGL((corner_kernel<R, MASK>), dim3(nb), dim3(256), 0, s, a);
+            if (nb > 0) hipLaunchKernelGGL((corner_kernel<R, MASK, TTS, TTT>), dim3(nb), dim3(256), 0, s, a);
         };
         ks->magL = [](const MhdArgs& a, int nb, hipStream_t s) {
             if (nb > 0) hipLaunchKernelGGL((magL_kernel<MASK>), dim3(nb), dim3(256), 0, s, a);
         };
         ks->bface = [](const MhdArgs& a, int nb, hipStream_t s) {
-            if (nb > 0) hipLaunchKernelGGL((bface_kernel<R, MASK>), dim3(nb), dim3(256), 0, s, a);
+            if (nb > 0) hipLaunchKernelGGL((bface_kernel<R, MASK, TTS, TTT>), dim3(nb), dim3(256), 0, s, a);
         };
         ks->refine = [](const MhdArgs& a, int nb, hipStream_t s) {
-            if (nb > 0) hipLaunchKernelGGL((refine_kernel<MASK>), dim3(nb), dim3(256), 0, s, a);
+            if (nb > 0) hipLaunchKernelGGL((refine_kernel<MASK, TTS, TTT>), dim3(nb), dim3(256), 0, s, a);
         };
         ks->copyB = [](const MhdArgs& a, double* dst, int nb, hipStream_t s) {
             if (nb > 0) hipLaunchKernelGGL((copyB_kernel<MASK>), dim3(nb), dim3(256), 0, s, a, dst);

@@ -120,25 +120,31 @@ __device__ inline void corner_speeds(const V8<R>& plus, const V8<R>& minus, int 
 }
 
 // corner EMF (mag_field.py:125-187); one thread per corner (i, j)
-template <class R, int MASK>
+template <class R, int MASK, int TS, int TT>
 __global__ __launch_bounds__(256) void corner_kernel(MhdArgs a) {
+    static_assert(TT == 8 && TS % 8 == 0, "one 8 x 8 block of cells per wave");
     using V = VS<MASK>;
-    const int ntiles = wave_uniform(*a.ntiles_dev);
+    // cell tiles of the y sweep (s = y, t = x): each wave takes an 8 x 8
+    // block, so the x-sweep arrays ([y][x]) and the y-sweep arrays and e3
+    // ([x][y]) are both read in 64-byte rows
+    const int ntiles = wave_uniform(*a.nttiles_dev[1]);
     const R* Dx = reinterpret_cast<const R*>(a.wD[0]);
     const R* Ux = reinterpret_cast<const R*>(a.wU[0]);
     const R* Dy = reinterpret_cast<const R*>(a.wD[1]);
     const R* Uy = reinterpret_cast<const R*>(a.wU[1]);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = ld_const(a.tiles + ti);
+        const Tile tl = ld_const(a.ttiles[1] + ti);
         const ProbState& st = a.ps[tl.prob];
         if (!ld_const(&st.active)) continue;
         const ProbDesc pd = ld_const(a.pd + tl.prob);
         const int n0 = pd.n0, n1 = pd.n1;
         const R gamma = R(pd.gamma);
-        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
-            const int cell = tl.s0 + c;
-            const int i = cell / n1, j = cell % n1;
+        for (int k = threadIdx.x; k < TS * TT; k += blockDim.x) {
+            const int r = k & 63;
+            const int i = tl.t0 + (r >> 3), j = tl.s0 + 8 * (k >> 6) + (r & 7);
+            if (i >= n0 || j >= n1) continue;
+            const int cell = i * n1 + j;
             auto ld = [&](const R* arr, long long o) {
                 V8<R> w = zero8<R, MASK>();
 #pragma unroll
@@ -201,15 +207,17 @@ __global__ __launch_bounds__(256) void magL_kernel(MhdArgs a) {
 }
 
 // stage 0: B_x, B_y of the state := the sweeps' face values (evolvers.py:73-76)
-template <class R, int MASK>
+template <class R, int MASK, int TS, int TT>
 __global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
+    static_assert(TT == 8 && TS % 8 == 0, "one 8 x 8 block of cells per wave");
     using V = VS<MASK>;
-    const int ntiles = wave_uniform(*a.ntiles_dev);
+    // cell tiles of the y sweep, 8 x 8 cells per wave (Fx is [y][x], U and Fy [x][y])
+    const int ntiles = wave_uniform(*a.nttiles_dev[1]);
     const long long P = a.plane;
     const R* Fx = reinterpret_cast<const R*>(a.src[0]);
     const R* Fy = reinterpret_cast<const R*>(a.src[1]);
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = ld_const(a.tiles + ti);
+        const Tile tl = ld_const(a.ttiles[1] + ti);
         const ProbState& st = a.ps[tl.prob];
         if (!ld_const(&st.active)) continue;
         // the state's B is kept so that a step that fails in a later stage
@@ -217,9 +225,11 @@ __global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
         const bool failed = ld_const(&st.fail_flag) != 0;
         const ProbDesc pd = ld_const(a.pd + tl.prob);
         const int n0 = pd.n0, n1 = pd.n1;
-        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
-            const int cell = tl.s0 + c;
-            const int i = cell / n1, j = cell % n1;
+        for (int k = threadIdx.x; k < TS * TT; k += blockDim.x) {
+            const int r = k & 63;
+            const int i = tl.t0 + (r >> 3), j = tl.s0 + 8 * (k >> 6) + (r & 7);
+            if (i >= n0 || j >= n1) continue;
+            const int cell = i * n1 + j;
             const long long ox = V::slot(5) * P + pd.off + cell, oy = V::slot(6) * P + pd.off + cell;
             a.bsave[pd.off + cell] = a.U[ox];
             a.bsave[P + pd.off + cell] = a.U[oy];
@@ -231,49 +241,93 @@ __global__ __launch_bounds__(256) void bface_kernel(MhdArgs a) {
 }
 
 // inverse reconstruction of B after a stage (mag_field.py:191-210): writes
-// the refined Bx, By of register K into two scratch planes
-template <int MASK>
+// the refined Bx, By of register K into two scratch planes.  Per axis (sweep
+// frame: s along the axis, t across it): face-centred values along t
+// (fv.py:67-85, 'face'), face -> cell centre along s (mag_field.py:203), then
+// centre -> average along s and t (fv.py:67-85, 'cntr').  A tile of TT x TS
+// cells (x, y) stages both B planes and each intermediate once on the window
+// x in [x0-2, x0+TT+3), y in [y0-2, y0+TS+3); a padded value at window
+// position p is the one at e(p) (clamped for edge boundaries, the position
+// itself for periodic ones, whose window holds the wrapped values).
+template <int MASK, int TS, int TT>
 __global__ __launch_bounds__(256) void refine_kernel(MhdArgs a) {
     using V = VS<MASK>;
-    const int ntiles = wave_uniform(*a.ntiles_dev);
+    constexpr int WX = TT + 5, WY = TS + 5;
+    __shared__ double Bw[2][WX][WY];
+    __shared__ double fcw[2][WX][WY];
+    __shared__ double ccw[2][WX][WY];
+    const int ntiles = wave_uniform(*a.nttiles_dev[1]);
     const long long P = a.plane;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const Tile tl = a.tiles[ti];
+        const Tile tl = ld_const(a.ttiles[1] + ti);   // y-sweep tiles: s = y, t = x
         const ProbState& st = a.ps[tl.prob];
-        if (!st.active || st.fail_flag) continue;
-        const ProbDesc pd = a.pd[tl.prob];
+        if (!ld_const(&st.active) || ld_const(&st.fail_flag)) continue;
+        const ProbDesc pd = ld_const(a.pd + tl.prob);
         const int n0 = pd.n0, n1 = pd.n1, bc = pd.bc;
-        for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
-            const int cell = tl.s0 + c;
-            const int i = cell / n1, j = cell % n1;
-#pragma unroll
-            for (int ax = 0; ax < 2; ++ax) {
-                // sweep frame of axis ax: s along ax, t along the other axis
-                const int ns = ax == 0 ? n0 : n1, nt = ax == 0 ? n1 : n0;
-                const int s = ax == 0 ? i : j, t = ax == 0 ? j : i;
-                const double* B = a.K + V::slot(5 + ax) * P + pd.off;
-                auto Bat = [&](int ss, int tt) {
-                    const int gs = bcmap(ss, ns, bc), gt = bcmap(tt, nt, bc);
-                    return ax == 0 ? B[(long long)gs * n1 + gt] : B[(long long)gt * n1 + gs];
-                };
-                // face-centred values: x - lap_t(x)/24 (fv.py:67-85, 'face')
-                auto fc = [&](int ss, int tt) {
-                    const int gs = bcmap(ss, ns, bc), gt = bcmap(tt, nt, bc);
-                    const double x0 = Bat(gs, gt);
-                    return x0 - K24 * ((Bat(gs, gt + 1) - x0) - (x0 - Bat(gs, gt - 1)));
-                };
-                // face-centred -> cell-centred (mag_field.py:203)
-                auto cc = [&](int ss, int tt) {
-                    const int gs = bcmap(ss, ns, bc), gt = bcmap(tt, nt, bc);
-                    return -1.0 / 16.0 * (fc(gs - 1, gt) + fc(gs + 2, gt)) + 9.0 / 16.0 * (fc(gs, gt) + fc(gs + 1, gt));
-                };
-                // cell-centred -> cell-averaged, both axes of the sweep frame (fv.py:67-85, 'cntr')
-                const double c0 = cc(s, t);
-                double r = c0 + K24 * ((cc(s + 1, t) - c0) - (c0 - cc(s - 1, t)));
-                r = r + K24 * ((cc(s, t + 1) - c0) - (c0 - cc(s, t - 1)));
-                a.scratch[ax * P + pd.off + cell] = r;
+        const int bx = tl.t0 - 2, by = tl.s0 - 2;
+        // canonical window index of window index w (e(p) of its position)
+        auto cx = [&](int w) { return bc == EDGE ? clampi(bx + w, 0, n0 - 1) - bx : w; };
+        auto cy = [&](int w) { return bc == EDGE ? clampi(by + w, 0, n1 - 1) - by : w; };
+        const double* B0 = a.K + V::slot(5) * P + pd.off;
+        const double* B1 = a.K + V::slot(6) * P + pd.off;
+        for (int k = threadIdx.x; k < WX * WY; k += blockDim.x) {
+            const int wx = k / WY, wy = k % WY;
+            const long long g = (long long)bcmap(bx + wx, n0, bc) * n1 + bcmap(by + wy, n1, bc);
+            Bw[0][wx][wy] = B0[g];
+            Bw[1][wx][wy] = B1[g];
+        }
+        __syncthreads();
+        // face-centred: axis 0 along y (t), axis 1 along x (t)
+        for (int k = threadIdx.x; k < WX * WY; k += blockDim.x) {
+            const int wx = k / WY, wy = k % WY;
+            if (wy >= 1 && wy < WY - 1) {
+                const int qx = cx(wx), qy = cy(wy);
+                const double x0 = Bw[0][qx][qy];
+                fcw[0][wx][wy] = x0 - K24 * ((Bw[0][qx][cy(qy + 1)] - x0) - (x0 - Bw[0][qx][cy(qy - 1)]));
+            }
+            if (wx >= 1 && wx < WX - 1) {
+                const int qx = cx(wx), qy = cy(wy);
+                const double x0 = Bw[1][qx][qy];
+                fcw[1][wx][wy] = x0 - K24 * ((Bw[1][cx(qx + 1)][qy] - x0) - (x0 - Bw[1][cx(qx - 1)][qy]));
             }
         }
+        __syncthreads();
+        // face -> cell centre along s: axis 0 along x, axis 1 along y
+        for (int k = threadIdx.x; k < WX * WY; k += blockDim.x) {
+            const int wx = k / WY, wy = k % WY;
+            if (wx >= 1 && wx < WX - 2 && wy >= 1 && wy < WY - 1) {
+                const int qx = cx(wx), qy = cy(wy);
+                ccw[0][wx][wy] = -1.0 / 16.0 * (fcw[0][cx(qx - 1)][qy] + fcw[0][cx(qx + 2)][qy]) +
+                                 9.0 / 16.0 * (fcw[0][qx][qy] + fcw[0][cx(qx + 1)][qy]);
+            }
+            if (wy >= 1 && wy < WY - 2 && wx >= 1 && wx < WX - 1) {
+                const int qx = cx(wx), qy = cy(wy);
+                ccw[1][wx][wy] = -1.0 / 16.0 * (fcw[1][qx][cy(qy - 1)] + fcw[1][qx][cy(qy + 2)]) +
+                                 9.0 / 16.0 * (fcw[1][qx][qy] + fcw[1][qx][cy(qy + 1)]);
+            }
+        }
+        __syncthreads();
+        // centre -> average: first along s, then along t
+        for (int k = threadIdx.x; k < TS * TT; k += blockDim.x) {
+            const int ix = k / TS, iy = k % TS;
+            const int i = tl.t0 + ix, j = tl.s0 + iy;
+            if (i >= n0 || j >= n1) continue;
+            const int wx = ix + 2, wy = iy + 2;
+            const long long o = pd.off + (long long)i * n1 + j;
+            {
+                const double c0 = ccw[0][wx][wy];
+                double r = c0 + K24 * ((ccw[0][cx(wx + 1)][wy] - c0) - (c0 - ccw[0][cx(wx - 1)][wy]));
+                r = r + K24 * ((ccw[0][wx][cy(wy + 1)] - c0) - (c0 - ccw[0][wx][cy(wy - 1)]));
+                a.scratch[o] = r;
+            }
+            {
+                const double c0 = ccw[1][wx][wy];
+                double r = c0 + K24 * ((ccw[1][wx][cy(wy + 1)] - c0) - (c0 - ccw[1][wx][cy(wy - 1)]));
+                r = r + K24 * ((ccw[1][cx(wx + 1)][wy] - c0) - (c0 - ccw[1][cx(wx - 1)][wy]));
+                a.scratch[P + o] = r;
+            }
+        }
+        __syncthreads();  // the next tile reuses the windows
     }
 }
 

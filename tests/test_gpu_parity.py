@@ -360,6 +360,34 @@ def test_mhd_ragged_tiles():
     assert e <= 1e-12
 
 
+@pytest.mark.parametrize("n,subgrid,solver", [(17, "weno5", "hlld"), (24, "ppm", "hlld"), (20, "plm", "llf")])
+def test_mhd_edge_boundary(n, subgrid, solver):
+    """The constrained-transport path with edge (outlet) boundaries, whose
+    padded intermediates are clamped rather than wrapped: Orszag-Tang data,
+    5 SSPRK(3,3) steps, device vs oracle."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    q0 = ic.initial_average(torch.from_numpy(ic.orszag_tang(n)), 5 / 3, "edge", True).numpy()
+    _say(f"[gpu] mhd edge boundary {n}^2 {subgrid} {solver}")
+    plan = Plan(2, subgrid, solver, "ssprk(3,3)", [dict(n0=n, n1=n, boundary="edge", dx=1 / n, gamma=5 / 3,
+                                                        cfl=.4, t_end=1e9, max_steps=5)], varmask=0x77,
+                magnetic_2d=True)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=5)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver=solver, boundary="edge", gamma=5 / 3, cfl=.4, dx=1 / n,
+                 magnetic_2d=True)
+    with np.errstate(all="ignore"):
+        ref = O.run(q0, s, 1e9, max_steps=5)
+    assert st["steps"] == ref.steps == 5 and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
 def _shear(n, seed=20261202):
     from astrea_amd import ic
     import torch
