@@ -22,7 +22,10 @@ template <class R, int MASK, int RECON, int SOLVER, bool SPLIT>
 inline void set2d(KernelSet* ks) {
     constexpr int CONV = RECON == PCM ? 0 : (RECON == PLM ? 1 : 2);
     constexpr int NVB = VS<MASK>::NV * (int)sizeof(R);
-    constexpr int RTS0 = 32, RTT0 = 8, RTS1 = 64, RTT1 = 4;
+    // reconstruction tiles; with six or eight FP64 components (MHD) the
+    // windows of 32 x 8 / 64 x 4 tiles take 56 KB of LDS (two workgroups per
+    // CU), those of 32 x 7 / 64 x 3 under 53 KB (three)
+    constexpr int RTS0 = 32, RTT0 = NVB > 32 ? 7 : 8, RTS1 = 64, RTT1 = NVB > 32 ? 3 : 4;
     // flux tile: (FTT + 2) x (FTS + 1) face solves and (FTT + 2) x (FTS + 2)
     // radii per tile each fit one pass of 256 threads; hydro 16 x 10 takes
     // 50.5 KB of LDS (three workgroups per CU), MHD 16 x 8 64 KB (two, as its
