@@ -1,4 +1,4 @@
-"""Continuous-batching solve server.
+"""Continuous-batching solve server for streams of 2D four-quadrant requests.
 
 Requests (2D problems of different sizes, schemes and end times) arrive over
 time.  Admitted requests keep their state resident on the device; requests

@@ -1,4 +1,5 @@
-"""Benchmark workloads (4 and 5) as reusable runners.
+"""Benchmark workloads (2D shear layer, ragged 1D Riemann ensemble, served
+four-quadrant stream) as reusable runners.
 
 Each runner builds its synthetic input on the device from a stated seed,
 runs the device update and returns a dict of measurements.  bench.py prints
@@ -30,7 +31,7 @@ def rel_l1(a, b):
 
 
 # ---------------------------------------------------------------------------
-# Config 4: ragged 1D Riemann ensemble
+# Ragged 1D Riemann ensemble
 # ---------------------------------------------------------------------------
 def ensemble_inputs(nprob=512, seed=SEED, nmin=256, nmax=8192, t_scale=1.0):
     specs = ic.ensemble_1d(nprob, seed, nmin, nmax)
@@ -104,7 +105,7 @@ def autotune_blocks(grids, probs, precision, candidates=(1024, 2048, 4096, 8192)
 
 
 # ---------------------------------------------------------------------------
-# Config 2: 2D shear layer (see bench.py)
+# 2D shear layer (see bench.py)
 # ---------------------------------------------------------------------------
 def shear_inputs(n, seed=SEED):
     return ic.to_device(ic.shear_layer(n, seed), 1.4, "wrap", True)

@@ -113,7 +113,7 @@ print(json.dumps(dict(seconds=dt, zu=r.steps * {n} * {n})))
 
 
 def bench_ensemble(args):
-    """ragged 1D Riemann ensemble, batched in one plan
+    """Ragged 1D Riemann ensemble (512 problems), batched in one plan
     (persistent tile loops, hipGraph-captured chunks, compaction) against the
     plain one-problem-at-a-time FP64 loop on a seeded sample."""
     import numpy as np
@@ -165,7 +165,7 @@ def bench_ensemble(args):
 
 
 def bench_serve(args):
-    """stream of 2D four-quadrant requests served with
+    """Stream of 180 2D four-quadrant requests served with
     continuous batching (FP64 and split), against the plain FIFO FP64 loop on
     the same requests.  Throughput comes from the saturated stream (every
     request queued at t=0, so zone-updates/s measures the server, not the
@@ -250,7 +250,7 @@ def div_b_max(plan, n):
 
 
 def bench_mhd(args):
-    """2D ideal-MHD Orszag-Tang vortex on [0,1]^2, periodic,
+    """2D ideal-MHD Orszag-Tang vortex on [0,1]^2 at 2048^2, periodic,
     gamma 5/3, WENO5, HLLD, constrained transport (mag_field.py), SSPRK(3,3),
     CFL 0.4, in FP64 (the constrained-transport update has no split variant).
     Reports zone-updates/s, the dominant kernel's roofline and max |div B|.
@@ -338,8 +338,8 @@ def bench_copy(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a design "
-              "note in DESIGN.md, not a measured path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the update is measured on one GPU's "
+              "share; sharding across GPUs is not part of this build", file=sys.stderr)
         sys.exit(2)
     if args.config == "copy":
         print(json.dumps(bench_copy(args)))

@@ -110,7 +110,7 @@ def rand_1d_prim(rng, N, mhd=False):
 
 
 def khi_prim(rng, N):
-    """2D shear layer (BASELINE config 2 initial state) on an N x N grid."""
+    """2D shear layer initial state on an N x N grid."""
     c = (np.arange(N) + .5) / N
     x, y = np.meshgrid(c, c, indexing='ij')
     w = np.zeros((N, N, 8))
@@ -125,7 +125,7 @@ def khi_prim(rng, N):
 
 
 def quadrant_prim(rng, N):
-    """2D four-quadrant Riemann problem with seeded states (BASELINE config 5)."""
+    """2D four-quadrant Riemann problem with seeded states."""
     c = (np.arange(N) + .5) / N
     x, y = np.meshgrid(c, c, indexing='ij')
     w = np.zeros((N, N, 8))
@@ -139,7 +139,7 @@ def quadrant_prim(rng, N):
 
 
 def ot_prim(N):
-    """Orszag-Tang vortex on [0,1]^2 (BASELINE config 3)."""
+    """Orszag-Tang vortex on [0,1]^2."""
     c = (np.arange(N) + .5) / N
     x, y = np.meshgrid(c, c, indexing='ij')
     w = np.zeros((N, N, 8))
