@@ -68,7 +68,10 @@ def _progress(msg):
 def cpu_baseline(n, steps, problem="shear"):
     """The oracle (numpy restatement of the reference, with the reference's
     LAPACK eigenvalue solve per interface) on a bounded sample of the same
-    workload, single-threaded BLAS, in a child process."""
+    workload, in a child process: once pinned to one core with one BLAS
+    thread (``value``), once on all the process's cores (at most 16) with
+    that many BLAS threads (``all_cores``; numpy's elementwise work stays
+    single-threaded either way)."""
     if problem == "shear":
         setup = f"""w = ic.shear_layer({n}, {SEED})
 q = ic.initial_average(torch.from_numpy(w), 1.4, 'wrap', True).numpy()
@@ -80,7 +83,9 @@ q = ic.initial_average(torch.from_numpy(w), 5 / 3, 'wrap', True).numpy()
 s = O.Scheme(dimension=2, subgrid='weno5', solver='hlld', timestep='ssprk(3,3)', boundary='wrap', gamma=5 / 3,
              cfl=.4, dx=1/{n}, magnetic_2d=True, eig='lapack', compat='physical')"""
     code = f"""
-import sys, time, json
+import os, sys, time, json
+if os.environ.get("AH_PIN_CPU"):
+    os.sched_setaffinity(0, {{int(os.environ["AH_PIN_CPU"])}})
 sys.path.insert(0, {ROOT!r})
 import numpy as np
 from oracle import astrea_oracle as O
@@ -92,12 +97,23 @@ r = O.run(q, s, 1e9, max_steps={steps})
 dt = time.perf_counter() - t0
 print(json.dumps(dict(seconds=dt, zu=r.steps * {n} * {n})))
 """
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
-               PYTHONWARNINGS="ignore")
-    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
-    if out.returncode != 0:
+    def run(threads, pin):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
+                   MKL_NUM_THREADS=str(threads), PYTHONWARNINGS="ignore")
+        env.pop("AH_PIN_CPU", None)
+        if pin is not None:
+            env["AH_PIN_CPU"] = str(pin)
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+        if out.returncode != 0:
+            return None
+        return json.loads(out.stdout.strip().splitlines()[-1])
+
+    cpus = sorted(os.sched_getaffinity(0))
+    r = run(1, cpus[0])
+    if r is None:
         return None
-    r = json.loads(out.stdout.strip().splitlines()[-1])
+    nall = min(len(cpus), 16)
+    r_all = run(nall, None)
     cpu = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -109,7 +125,10 @@ print(json.dumps(dict(seconds=dt, zu=r.steps * {n} * {n})))
         pass
     return dict(value=r["zu"] / r["seconds"], unit="zone-updates/s", cores=1, kind="port",
                 sample=f"oracle (numpy restatement, LAPACK eigvals per interface) on the same {problem} "
-                       f"problem at {n}x{n}, {steps} steps, {r['seconds']:.1f} s, 1 BLAS thread, CPU: {cpu}")
+                       f"problem at {n}x{n}, {steps} steps, {r['seconds']:.1f} s, pinned to one core with one "
+                       f"BLAS thread, CPU: {cpu}",
+                all_cores=None if r_all is None else dict(value=r_all["zu"] / r_all["seconds"], cores=nall,
+                                                          seconds=r_all["seconds"]))
 
 
 def bench_ensemble(args):
