@@ -80,6 +80,23 @@ template <class R>
 __device__ inline R nmax(R a, R b) { return (a > b || a != a) ? a : b; }  // np.maximum
 template <class R>
 __device__ inline R nsign(R x) { return x > R(0) ? R(1) : (x < R(0) ? R(-1) : x); }
+// x / dx of the flux differences.  When dx is a power of two whose
+// reciprocal is representable, x * (1/dx) and x / dx are the correctly
+// rounded values of the same real number, hence equal for every x (the
+// division is replaced by a multiplication); otherwise it divides.
+struct DivDx {
+    double dx, inv;
+    bool pow2;
+    __device__ explicit DivDx(double d) : dx(d) {
+        int e = 0;
+        const double m = frexp(d, &e);  // d = m 2^e, m in [0.5, 1)
+        pow2 = m == 0.5 && 1 - e <= 1023;
+        inv = pow2 ? ldexp(1.0, 1 - e) : 0.0;
+    }
+    template <class R>
+    __device__ R operator()(R x) const { return pow2 ? x * R(inv) : x / R(dx); }
+};
+
 // Hardware min/max (v_min_f64 / v_max_f64).  On numbers they equal
 // np.minimum / np.maximum except in the sign of a zero result; given a NaN
 // they return the other operand where numpy returns NaN.  Used only where the

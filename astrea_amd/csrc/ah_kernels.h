@@ -557,13 +557,13 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     }
     __syncthreads();
     // F: L = -(F[i+1] - F[i]) / dx (evolvers.py:41-60)
-    const R dxR = R(pd.dx);
+    const DivDx bydx(pd.dx);
     for (int i = threadIdx.x; i < TS; i += blockDim.x) {
         const int cell = s0 + i;
         if (cell >= N) break;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const R d = (fl[v][i + 1] - fl[v][i]) / dxR;
+            const R d = bydx(R(fl[v][i + 1] - fl[v][i]));
             a.out[v * P + off + cell] = -double(d);
         }
     }
@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
         __syncthreads();
     }
     // E: flux difference; first sweep writes D, second writes L = -(D0 + D1)
-    const R dxR = R(pd.dx);
+    const DivDx bydx(pd.dx);
     for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
         int itt, is;
         if constexpr (AX == 0) { is = i / TT; itt = i % TT; }
@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
         const long long g = gidx(s, t);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const double d = double((F_s[v][itt][is + 1] - F_s[v][itt][is]) / dxR);
+            const double d = double(bydx(R(F_s[v][itt][is + 1] - F_s[v][itt][is])));
             if (a.last)
                 a.out[v * P + g] = -(a.Dprev[v * P + g] + d);
             else

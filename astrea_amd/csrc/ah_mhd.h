@@ -192,6 +192,7 @@ __global__ __launch_bounds__(256) void magL_kernel(MhdArgs a) {
         const Tile tl = ld_const(a.tiles + ti);
         if (!ld_const(&a.ps[tl.prob].active)) continue;
         const ProbDesc pd = ld_const(a.pd + tl.prob);
+        const DivDx bydx(pd.dx);
         const int n0 = pd.n0, n1 = pd.n1;
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x) {
             const int cell = tl.s0 + c;
@@ -200,8 +201,8 @@ __global__ __launch_bounds__(256) void magL_kernel(MhdArgs a) {
             const double e0 = e[cell];
             const double dy = e[(long long)i * n1 + bcmap(j + 1, n1, pd.bc)] - e0;
             const double dxd = e[(long long)bcmap(i + 1, n0, pd.bc) * n1 + j] - e0;
-            a.L[V::slot(5) * P + pd.off + cell] = -(dy / pd.dx);
-            a.L[V::slot(6) * P + pd.off + cell] = dxd / pd.dx;
+            a.L[V::slot(5) * P + pd.off + cell] = -bydx(dy);
+            a.L[V::slot(6) * P + pd.off + cell] = bydx(dxd);
         }
     }
 }
