@@ -25,7 +25,9 @@ AH_VARS_H1, AH_VARS_H2, AH_VARS_M6, AH_VARS_ALL = 0x13, 0x17, 0x77, 0xFF
 EXPORTS = ["ah_version", "ah_status_str", "ah_plan_create", "ah_plan_destroy", "ah_plan_layout",
            "ah_plan_workspace_size", "ah_plan_bind", "ah_plan_reset", "ah_plan_step", "ah_plan_status",
            "ah_evolve_space", "ah_compute_L", "ah_plan_profile", "ah_plan_profile_read", "ah_plan_step_graph",
-           "ah_plan_compact", "ah_plan_set_blocks", "ah_plan_time_export", "ah_plan_time_import"]
+           "ah_plan_compact", "ah_plan_set_blocks", "ah_plan_time_export", "ah_plan_time_import",
+           "ah_plan_init_ic"]
+AH_IC_RIEMANN_1D, AH_IC_QUADRANTS_2D = 0, 1
 KERNEL_KINDS = ["sweep1d", "recon_ax0", "flux_ax0", "recon_ax1", "flux_ax1", "combine", "dt", "advance"]
 
 
@@ -51,6 +53,11 @@ class AhProblemStatus(ctypes.Structure):
 class AhTimeState(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("steps", ctypes.c_longlong),
                 ("idx", ctypes.c_int), ("active", ctypes.c_int), ("failed", ctypes.c_int), ("parity", ctypes.c_int)]
+
+
+class AhIc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("high_order", ctypes.c_int), ("x0", ctypes.c_double),
+                ("y0", ctypes.c_double), ("w", (ctypes.c_double * 8) * 4)]
 
 
 class AstreaHipError(RuntimeError):
@@ -90,6 +97,7 @@ def load():
     lib.ah_plan_set_blocks.argtypes = [vp, i]
     lib.ah_plan_time_export.argtypes = [vp, P(AhTimeState), vp]
     lib.ah_plan_time_import.argtypes = [vp, P(AhTimeState), vp]
+    lib.ah_plan_init_ic.argtypes = [vp, i, P(AhIc), vp]
     lib.ah_plan_profile_read.argtypes = [vp, P(d), P(i)]
     for name in EXPORTS:
         if name != "ah_status_str":

@@ -9,6 +9,7 @@
 
 #include "../../include/astrea_hip.h"
 #include "ah_control.h"
+#include "ah_ic.h"
 #include "ah_launch.h"
 
 using namespace ah;
@@ -577,6 +578,35 @@ int ah_plan_step_graph(ah_plan* p, int nsteps, int nlaunch, void* stream) {
         p->graph_steps = nsteps;
     }
     for (int i = 0; i < nlaunch; ++i) HIPCHK(hipGraphLaunch(p->graph_exec, s));
+    return AH_OK;
+}
+
+int ah_plan_init_ic(ah_plan* p, int prob, const ah_ic* ic, void* stream) {
+    if (!p || !ic || prob < 0 || prob >= p->nprob) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    if (ic->kind != AH_IC_RIEMANN_1D && ic->kind != AH_IC_QUADRANTS_2D) return AH_ERR_INVALID;
+    if ((ic->kind == AH_IC_RIEMANN_1D) != (p->sch.dims == 1)) return AH_ERR_INVALID;
+    const ah_problem& q = p->probs[prob];
+    IcArgs a;
+    memset(&a, 0, sizeof(a));
+    a.state = p->U;
+    a.plane = p->total;
+    a.off = p->offs[prob];
+    a.n0 = q.n0;
+    a.n1 = p->sch.dims == 1 ? 1 : q.n1;
+    a.dims = p->sch.dims;
+    a.bc = q.boundary == AH_WRAP ? WRAP : EDGE;
+    a.nv = p->nv;
+    for (int v = 0; v < p->nv; ++v) a.varidx[v] = p->varidx[v];
+    a.kind = ic->kind;
+    a.high_order = ic->high_order != 0;
+    a.gamma = q.gamma;
+    a.x0 = ic->x0;
+    a.y0 = ic->y0;
+    memcpy(a.w, ic->w, sizeof(a.w));
+    const long long cells = (long long)a.n0 * a.n1;
+    hipLaunchKernelGGL(ic_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
     return AH_OK;
 }
 

@@ -163,6 +163,34 @@ class Plan:
             arr[i] = L.AhTimeState(*s)
         L.check(self.lib.ah_plan_time_import(self.h, arr, self._stream()), "ah_plan_time_import")
 
+    # ---- initial states built on the device (ah_plan_init_ic) ---------------
+    def _high_order(self):
+        return self.scheme.recon in (L.AH_PPM, L.AH_WENO3, L.AH_WENO5, L.AH_WENO7)
+
+    def _init_ic(self, i, kind, regions, x0, y0, high_order):
+        ic = L.AhIc()
+        ic.kind = kind
+        ic.high_order = int(self._high_order() if high_order is None else high_order)
+        ic.x0, ic.y0 = float(x0), float(y0)
+        for k, w in enumerate(regions):
+            for c in range(8):
+                ic.w[k][c] = float(w[c])
+        L.check(self.lib.ah_plan_init_ic(self.h, int(i), ctypes.byref(ic), self._stream()), "ah_plan_init_ic")
+
+    def init_riemann(self, i, left, right, x0=.5, high_order=None):
+        """Problem i (1D) starts from the Riemann data left = (rho, u, p) for
+        x <= x0 and right otherwise, built on the device (constructor.py:11-109);
+        high_order defaults to the plan's reconstruction (PPM/WENO)."""
+        reg = [[l[0], l[1], 0, 0, l[2], 0, 0, 0] for l in (left, right)]
+        self._init_ic(i, L.AH_IC_RIEMANN_1D, reg, x0, 0., high_order)
+
+    def init_quadrants(self, i, states, x0=.5, y0=.5, high_order=None):
+        """Problem i (2D) starts from four-quadrant data, states[k] = (rho, u,
+        v, p) for k = 0 (x <= x0, y < y0), 1 (x > x0, y < y0), 2 (x <= x0,
+        y >= y0), 3 (x > x0, y >= y0), built on the device."""
+        reg = [[s[0], s[1], s[2], 0, s[3], 0, 0, 0] for s in states]
+        self._init_ic(i, L.AH_IC_QUADRANTS_2D, reg, x0, y0, high_order)
+
     def bytes_per_cell(self):
         """Device bytes per cell of this plan (state + workspace)."""
         return (self.state.numel() * 8 + self.workspace.numel()) / max(self.total, 1)

@@ -166,6 +166,29 @@ int ah_plan_compact(ah_plan* plan, int* nactive, void* stream);
 /* Persistent grid cap: workgroups per launch (each loops over its tiles). */
 int ah_plan_set_blocks(ah_plan* plan, int max_blocks);
 
+/* Initial state of problem `prob` built on the device straight into the
+ * bound state planes, replacing constructor.initialise
+ * (functions/constructor.py:11-109) for piecewise-constant problems:
+ * primitive states at the cell centres of [0,1]^d (constructor.py:13-16),
+ * then the conservative cell averages of constructor.py:105-109 (4th-order
+ * conversion when `high_order`, i.e. for PPM and WENO, then the h^2/24
+ * Laplacian), equal to that pipeline to the last bit.
+ *   AH_IC_RIEMANN_1D:   w[0] for x <= x0, w[1] otherwise;
+ *   AH_IC_QUADRANTS_2D: w[0] for x <= x0 and y < y0, w[1] for x > x0 and
+ *                       y < y0, w[2] for x <= x0 and y >= y0, w[3] otherwise
+ *                       (the Lax-Liu layout of constructor.py:57-60).
+ * w[k] are primitive 8-vectors (rho, vx, vy, vz, p, Bx, By, Bz); components
+ * the plan does not store must be zero.  The serving loop admits requests
+ * without shipping their grids. */
+enum ah_ic_kind { AH_IC_RIEMANN_1D = 0, AH_IC_QUADRANTS_2D = 1 };
+typedef struct ah_ic {
+    int kind;        /* ah_ic_kind */
+    int high_order;  /* 1: 4th-order averaging (PPM, WENO); 0: pointwise conversion */
+    double x0, y0;   /* region split positions */
+    double w[4][8];  /* primitive state per region */
+} ah_ic;
+int ah_plan_init_ic(ah_plan* plan, int prob, const ah_ic* ic, void* stream);
+
 /* Export / import the time state of all problems of a plan (synchronous),
  * so a problem can move between plans with its state planes: the serving
  * loop re-plans its batch as requests arrive and finish.  No reference

@@ -37,6 +37,7 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(L.AhScheme) == 16 * 4
     assert ctypes.sizeof(L.AhProblem) == 4 * 4 + 4 * 8 + 8
     assert ctypes.sizeof(L.AhProblemStatus) == 8 * 3 + 2 * 4
+    assert ctypes.sizeof(L.AhIc) == 2 * 4 + 2 * 8 + 32 * 8
 
 
 @pytest.mark.parametrize("s,expect", [("ppm", L.AH_PPM), ("parabolic", L.AH_PPM), ("polynomial", L.AH_PCM),

@@ -250,3 +250,26 @@ def test_ssprk33_matches_butcher_form():
     ref = q + dt * (k1 + k2 + 4 * k3) / 6
     out = O.evolve_time(q, O.evolve_space(q, s), dt, s)
     np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("case", ["sod", "quad_edge", "quad_wrap", "ot", "shear"])
+@pytest.mark.parametrize("subgrid", ["plm", "ppm"])
+def test_torch_initial_state_equals_oracle(case, subgrid):
+    """astrea_amd.ic's torch pipeline (the initial state of the benchmark
+    and serving workloads, and the reference for the device IC kernel) is
+    the oracle's constructor restatement to the last bit."""
+    import torch
+    from astrea_amd import ic
+    if case == "sod":
+        w, dims, bc, gamma = ic.sod(37), 1, "edge", 1.4
+    elif case.startswith("quad"):
+        w = ic.quadrants(21, [(1, .2, .1, 1), (.5, -.3, .2, .4), (.2, .1, -.5, .2), (.8, 0, 0, .9)])
+        dims, bc, gamma = 2, case[5:], 1.4
+    elif case == "ot":
+        w, dims, bc, gamma = ic.orszag_tang(16), 2, "wrap", 5 / 3
+    else:
+        w, dims, bc, gamma = ic.shear_layer(24, 5), 2, "wrap", 1.4
+    s = O.Scheme(dimension=dims, subgrid=subgrid, boundary=bc, gamma=gamma)
+    ref = O.initial_average(w, s)
+    got = ic.initial_average(torch.from_numpy(w), gamma, bc, subgrid == "ppm").numpy()
+    assert np.array_equal(got, ref)
