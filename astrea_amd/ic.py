@@ -1,13 +1,17 @@
-"""Initial conditions on the device.
+"""Initial conditions.
 
 The reference builds a pointwise primitive state, converts it to
 conservative variables and adds the h^2/24 Laplacian for every scheme
 (functions/constructor.py:105-109); the conversion is the 4th-order one for
 PPM/WENO and the pointwise one otherwise (generic.py:250-255).  Here the same
-pipeline runs in FP64 torch on the GPU, each elementwise operation rounding
-once exactly as numpy does.  Generators for the benchmark problems draw
-their random numbers on the host with numpy's PCG64 (seed stated by the
-caller) so every run is reproducible.
+pipeline runs in FP64 torch on the host, each elementwise operation rounding
+once exactly as numpy does (pinned against the oracle's restatement), and
+``to_device`` copies the result.  torch's GPU elementwise kernels differ from
+numpy in the last bit on some cells, so the averaging does not run there;
+piecewise-constant problems are instead built on the device by
+ah_plan_init_ic, bit for bit this pipeline.  Generators for the benchmark
+problems draw their random numbers on the host with numpy's PCG64 (seed
+stated by the caller) so every run is reproducible.
 """
 import math
 
@@ -163,5 +167,6 @@ def quadrant_requests(nreq, seed, sizes=(128, 256, 512, 1024)):
 
 
 def to_device(w_point, gamma, bc, high_order, device="cuda:0"):
-    w = torch.as_tensor(w_point, dtype=torch.float64, device=device)
-    return initial_average(w, gamma, bc, high_order)
+    """Averaged conservative initial state (host pipeline), on ``device``."""
+    w = torch.as_tensor(w_point, dtype=torch.float64, device="cpu")
+    return initial_average(w, gamma, bc, high_order).to(device)

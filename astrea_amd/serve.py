@@ -33,8 +33,8 @@ class Request:
     subgrid: str
     arrival: float
     t_end: float = .25
-    states: list = None
-    grid: torch.Tensor = None          # initial conservative state (n, n, 8), device
+    states: list = None                # four-quadrant primitive states (rho, u, v, p)
+    grid: torch.Tensor = None          # or an explicit initial conservative state (n, n, 8)
     admitted_at: float = None
     done_at: float = None
     status: dict = None
@@ -46,8 +46,11 @@ class Request:
         return None if self.done_at is None else self.done_at - self.arrival
 
 
-def make_requests(nreq, seed, rate, sizes=(128, 256, 512, 1024), device="cuda:0"):
-    """Seeded four-quadrant requests with Poisson arrivals at ``rate`` per second."""
+def make_requests(nreq, seed, rate, sizes=(128, 256, 512, 1024), device="cuda:0", host_grids=False):
+    """Seeded four-quadrant requests with Poisson arrivals at ``rate`` per
+    second.  A request carries its four states only: its initial grid is
+    built on the device when it is admitted (ah_plan_init_ic).  With
+    ``host_grids`` each request also carries the host-pipeline grid."""
     import numpy as np
     specs = ic.quadrant_requests(nreq, seed, sizes)
     rng = np.random.default_rng(seed + 1)
@@ -55,10 +58,22 @@ def make_requests(nreq, seed, rate, sizes=(128, 256, 512, 1024), device="cuda:0"
     out = []
     for i, sp in enumerate(specs):
         t += float(rng.exponential(1.0 / rate))
-        w = ic.quadrants(sp["n"], sp["states"])
-        g = ic.to_device(w, 1.4, "edge", sp["subgrid"] == "ppm", device)
+        g = None
+        if host_grids:
+            g = ic.to_device(ic.quadrants(sp["n"], sp["states"]), 1.4, "edge", sp["subgrid"] == "ppm", device)
         out.append(Request(i, sp["n"], sp["subgrid"], t, .25, sp["states"], g))
     return out
+
+
+def _initial_state(plan, j, r):
+    """Problem j of ``plan`` starts from request r's initial state."""
+    if r.grid is None:
+        plan.init_quadrants(j, r.states)
+        return
+    o, cells = plan.offsets[j], r.n * r.n
+    flat = r.grid.reshape(-1, 8)
+    for v, c in enumerate(plan.varidx):
+        plan.state[v, o:o + cells] = flat[:, c]
 
 
 def _problem(r, cfl):
@@ -102,9 +117,7 @@ class _Group:
                     new.state[:, o_new:o_new + cells].copy_(old.state[:, o_old:o_old + cells])
                     times.append(old_times[i])
                 else:
-                    flat = r.grid.reshape(-1, 8)
-                    for v, c in enumerate(new.varidx):
-                        new.state[v, o_new:o_new + cells] = flat[:, c]
+                    _initial_state(new, j, r)
                     active = 1 if (0.0 <= r.t_end and r.t_end != 0.0) else 0
                     times.append((0.0, 0.0, 0, 1, active, 0, 0))
             new.time_import(times)
@@ -259,7 +272,7 @@ def run_fifo(requests, precision="fp64", compat="physical", cfl=.4, chunk=64, re
         r.admitted_at = now()
         plan = Plan(2, r.subgrid, "hllc", "ssprk(3,3)", [_problem(r, cfl)], precision=precision, varmask=0x17,
                     compat=compat, device=device)
-        plan.load([r.grid])
+        _initial_state(plan, 0, r)
         plan.reset()
         r.status = plan.run(chunk)[0]
         r.done_at = now()

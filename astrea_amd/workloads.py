@@ -33,22 +33,36 @@ def rel_l1(a, b):
 # ---------------------------------------------------------------------------
 # Ragged 1D Riemann ensemble
 # ---------------------------------------------------------------------------
-def ensemble_inputs(nprob=512, seed=SEED, nmin=256, nmax=8192, t_scale=1.0):
+def ensemble_inputs(nprob=512, seed=SEED, nmin=256, nmax=8192, t_scale=1.0, host_grids=False):
+    """Problems of the seeded ensemble and their initial data: the Riemann
+    states (built on the device when a plan starts, ah_plan_init_ic), or with
+    ``host_grids`` the host-pipeline grids."""
     specs = ic.ensemble_1d(nprob, seed, nmin, nmax)
-    grids, probs = [], []
+    inits, probs = [], []
     for sp in specs:
-        w = ic.riemann_1d(sp["n"], sp["left"], sp["right"])
-        grids.append(ic.to_device(w, 1.4, "edge", True))
+        if host_grids:
+            inits.append(ic.to_device(ic.riemann_1d(sp["n"], sp["left"], sp["right"]), 1.4, "edge", True))
+        else:
+            inits.append(dict(left=sp["left"], right=sp["right"]))
         probs.append(dict(n0=sp["n"], boundary="edge", dx=1 / sp["n"], gamma=1.4, cfl=.5,
                           t_end=sp["t_end"] * t_scale))
-    return grids, probs
+    return inits, probs
+
+
+def _start(plan, inits):
+    """Initial states: Riemann data built on the device, or grids loaded."""
+    if inits and isinstance(inits[0], dict):
+        for i, d in enumerate(inits):
+            plan.init_riemann(i, d["left"], d["right"])
+    else:
+        plan.load(inits)
 
 
 def run_ensemble_batched(grids, probs, precision="fp64", chunk=16, use_graph=True, blocks=None, compat="physical"):
     plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
     if blocks:
         plan.set_blocks(blocks)
-    plan.load(grids)
+    _start(plan, grids)
     plan.reset()
     torch.cuda.synchronize()
     st, secs, nch = batch.run(plan, chunk=chunk, use_graph=use_graph)
@@ -67,7 +81,7 @@ def run_ensemble_plain(grids, probs, sample=None, chunk=64, compat="physical"):
     zu, secs, st = 0, 0.0, {}
     for i in idx:
         plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", [probs[i]], varmask=0x13, compat=compat)
-        plan.load([grids[i]])
+        _start(plan, [grids[i]])
         plan.reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -87,7 +101,7 @@ def autotune_blocks(grids, probs, precision, candidates=(1024, 2048, 4096, 8192)
     for b in candidates:
         plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
         plan.set_blocks(b)
-        plan.load(grids)
+        _start(plan, grids)
         plan.reset()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())

@@ -20,6 +20,7 @@ DT = 0.05
 class FakePlan:
     live = 0          # plans alive at once (class-wide)
     max_cells = 0     # most cells held by plans alive at once
+    device_ic = 0     # initial states built by init_quadrants
 
     def __init__(self, dims, subgrid, solver, timestep, problems, **kw):
         self.problems = problems
@@ -61,6 +62,11 @@ class FakePlan:
             for v, c in enumerate(self.varidx):
                 self.state[v, o:o + cells] = flat[:, c]
 
+    def init_quadrants(self, i, states):
+        o, cells = self.offsets[i], self.problems[i]["n0"] * self.problems[i]["n1"]
+        FakePlan.device_ic += 1
+        self.state[:, o:o + cells] = 0.0
+
     def reset(self):
         self.times = [(0.0, 0.0, 0, 1, 1, 0, 0) for _ in self.problems]
 
@@ -89,7 +95,7 @@ class _Stream:
 
 @pytest.fixture
 def fake_gpu(monkeypatch):
-    FakePlan.live = FakePlan.max_cells = 0
+    FakePlan.live = FakePlan.max_cells = FakePlan.device_ic = 0
     monkeypatch.setattr(serve, "Plan", FakePlan)
     monkeypatch.setattr(torch.cuda, "Stream", _Stream)
     monkeypatch.setattr(torch.cuda, "stream", lambda s: contextlib.nullcontext())
@@ -161,5 +167,16 @@ def test_active_cell_cap_bounds_the_batch(fake_gpu, monkeypatch):
     out = srv.run(_requests(SPEC), realtime=False)
     assert out["requests"] == len(SPEC)
     assert seen and max(seen) <= cap
+    for r in out["done"]:
+        assert torch.all(r.result == r.status["steps"])
+
+
+def test_requests_without_grids_start_on_the_device(fake_gpu):
+    reqs = _requests(SPEC)
+    for r in reqs:
+        r.grid, r.states = None, [(1., 0., 0., 1.)] * 4
+    srv = serve.Server(hbm_budget=1 << 40, chunk=3, device="cpu", keep_results=True, bytes_per_cell=1)
+    out = srv.run(reqs, realtime=False)
+    assert out["requests"] == len(SPEC) and FakePlan.device_ic == len(SPEC)
     for r in out["done"]:
         assert torch.all(r.result == r.status["steps"])
