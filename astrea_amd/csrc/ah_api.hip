@@ -11,6 +11,7 @@
 #include "ah_control.h"
 #include "ah_ic.h"
 #include "ah_launch.h"
+#include "ah_output.h"
 
 using namespace ah;
 
@@ -145,6 +146,27 @@ struct ProfScope {
         if (on && hipEventRecord(p->ev[2 * p->ev_used + 1], s) == hipSuccess) p->ev_used++;
     }
 };
+}  // namespace
+
+namespace {
+template <int MASK>
+void launch_snapshot(const ah_plan* p, int prob, double* out, hipStream_t s) {
+    const ah_problem& q = p->probs[prob];
+    const int n1 = p->sch.dims == 1 ? 1 : q.n1;
+    const int bc = q.boundary == AH_WRAP ? WRAP : EDGE;
+    const bool ho = p->sch.recon == AH_PPM || p->sch.recon == AH_WENO3 || p->sch.recon == AH_WENO5 ||
+                    p->sch.recon == AH_WENO7;
+    const long long cells = (long long)q.n0 * n1;
+    const dim3 g((unsigned)((cells + 255) / 256)), b(256);
+    const long long off = p->offs[prob];
+    if (p->sch.dims == 1) {
+        if (ho) hipLaunchKernelGGL((snapshot_kernel<MASK, 1, true>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
+        else hipLaunchKernelGGL((snapshot_kernel<MASK, 1, false>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
+    } else {
+        if (ho) hipLaunchKernelGGL((snapshot_kernel<MASK, 2, true>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
+        else hipLaunchKernelGGL((snapshot_kernel<MASK, 2, false>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -606,6 +628,21 @@ int ah_plan_init_ic(ah_plan* p, int prob, const ah_ic* ic, void* stream) {
     memcpy(a.w, ic->w, sizeof(a.w));
     const long long cells = (long long)a.n0 * a.n1;
     hipLaunchKernelGGL(ic_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+int ah_plan_snapshot(ah_plan* p, int prob, double* out, void* stream) {
+    if (!p || !out || prob < 0 || prob >= p->nprob) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    switch (p->sch.varmask) {
+        case AH_VARS_H1: launch_snapshot<AH_VARS_H1>(p, prob, out, s); break;
+        case AH_VARS_H2: launch_snapshot<AH_VARS_H2>(p, prob, out, s); break;
+        case AH_VARS_M6: launch_snapshot<AH_VARS_M6>(p, prob, out, s); break;
+        case AH_VARS_ALL: launch_snapshot<AH_VARS_ALL>(p, prob, out, s); break;
+        default: return AH_ERR_UNSUPPORTED;
+    }
     HIPCHK(hipGetLastError());
     return AH_OK;
 }

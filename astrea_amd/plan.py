@@ -191,6 +191,19 @@ class Plan:
         reg = [[s[0], s[1], s[2], 0, s[3], 0, 0, 0] for s in states]
         self._init_ic(i, L.AH_IC_QUADRANTS_2D, reg, x0, y0, high_order)
 
+    def snapshot(self, i, out=None):
+        """Problem i's primitive snapshot as the reference saves it each step
+        (astrea.py:47): convert_conservative of the state, transposed by
+        ortho_axis; a device tensor of shape (n0, 8) in 1D, (n1, n0, 8) in 2D."""
+        shape = self.shape(i)[::-1] + (8,)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        if tuple(out.shape) != shape or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError(f"snapshot buffer must be a contiguous float64 tensor of shape {shape}")
+        L.check(self.lib.ah_plan_snapshot(self.h, int(i), ctypes.c_void_p(out.data_ptr()), self._stream()),
+                "ah_plan_snapshot")
+        return out
+
     def bytes_per_cell(self):
         """Device bytes per cell of this plan (state + workspace)."""
         return (self.state.numel() * 8 + self.workspace.numel()) / max(self.total, 1)

@@ -189,6 +189,16 @@ typedef struct ah_ic {
 } ah_ic;
 int ah_plan_init_ic(ah_plan* plan, int prob, const ah_ic* ic, void* stream);
 
+/* The primitive snapshot of problem `prob` as the reference saves it every
+ * step (astrea.py:47): sim_variables.convert_conservative of the state (the
+ * 4th-order cell conversion for PPM/WENO, the pointwise one otherwise,
+ * generic.py:250-255) transposed by ortho_axis (generic.py:264), all 8
+ * components, FP64: `out` (device memory, cells * 8 doubles) receives the
+ * array of shape (n0, 8) in 1D and (n1, n0, 8) in 2D, equal to the
+ * reference's to the last bit.  The HDF5 writing around it (astrea.py:48-50)
+ * stays with the caller. */
+int ah_plan_snapshot(ah_plan* plan, int prob, double* out, void* stream);
+
 /* Export / import the time state of all problems of a plan (synchronous),
  * so a problem can move between plans with its state planes: the serving
  * loop re-plans its batch as requests arrive and finish.  No reference
