@@ -253,21 +253,6 @@ def bench_serve(args):
     }
 
 
-def div_b_max(plan, n):
-    """max |div B| by cell-centred central differences on the periodic grid
-    (the reference has no div B diagnostic), and the same divided by
-    max|B| / dx."""
-    import torch
-    st = plan.state
-    bx = st[plan.varidx.index(5)].reshape(n, n)
-    by = st[plan.varidx.index(6)].reshape(n, n)
-    dx = 1.0 / n
-    div = (torch.roll(bx, -1, 0) - torch.roll(bx, 1, 0)) / (2 * dx) + (torch.roll(by, -1, 1) - torch.roll(by, 1, 1)) / (2 * dx)
-    bmax = float(torch.sqrt(bx * bx + by * by).max())
-    m = float(div.abs().max())
-    return m, m * dx / bmax
-
-
 def bench_mhd(args):
     """2D ideal-MHD Orszag-Tang vortex on [0,1]^2 at 2048^2, periodic,
     gamma 5/3, WENO5, HLLD, constrained transport (mag_field.py), SSPRK(3,3),
@@ -279,7 +264,7 @@ def bench_mhd(args):
     bit for bit up to there; the timed window therefore stays inside the
     first 16 steps (--mhd-steps, --mhd-warmup)."""
     import torch
-    from astrea_amd import ic
+    from astrea_amd import diagnostics as D, ic
     from astrea_amd.plan import Plan
     n = args.mhd_n
     args.steps, args.warmup = args.mhd_steps, args.mhd_warmup
@@ -289,7 +274,7 @@ def bench_mhd(args):
                 varmask=0x77, magnetic_2d=True, compat="physical")
     plan.load([q0])
     plan.reset()
-    div0 = div_b_max(plan, n)
+    div0 = D.div_b(plan, 0)
     plan.step(args.warmup)
     torch.cuda.synchronize()
     plan.profile(64 * args.steps + 8)
@@ -302,7 +287,7 @@ def bench_mhd(args):
     prof = plan.profile_read()
     st = plan.status()[0]
     assert st["steps"] == args.warmup + args.steps and not st["failed"], st
-    div1 = div_b_max(plan, n)
+    div1 = D.div_b(plan, 0)
     plan.close()
     zu = n * n * args.steps
     value = zu / (ms / 1e3)

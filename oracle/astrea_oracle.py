@@ -1160,3 +1160,54 @@ def initial_average(w_point, s):
     +h^2/24 Laplacian for every scheme."""
     q = to_cons(w_point, replace(s, parity=0), "cell")
     return ho_convert("cntr", q, s.boundary, range(s.dimension))
+
+
+# ----------------------------------------------------------------------------
+# Diagnostics of snapshots (functions/analytic.py); a snapshot is the
+# primitive array the reference stores each step (astrea.py:47)
+# ----------------------------------------------------------------------------
+def _divide(a, b):
+    """fv.divide (fv.py:19-20): a / b where b != 0, else 0."""
+    return np.divide(a, b, out=np.zeros_like(a), where=b != 0)
+
+
+def snapshot_energies(w, gamma):
+    """Specific total and internal energy of a primitive snapshot
+    (analytic.py:33-34, 54-55; fv.convert_variable('pressure'), fv.py:51)."""
+    e = w[..., 4] / (gamma - 1) + .5 * (w[..., 0] * norm2(w[..., 1:4]) + norm2(w[..., 5:8]))
+    return _divide(e, w[..., 0]), _divide(w[..., 4], w[..., 0] * (gamma - 1))
+
+
+def solution_error(w_num, w_theo, gamma, p):
+    """analytic.py:24-44 (10 components: the 8 primitives, E_tot, E_int)."""
+    dims = w_num.ndim - 1
+    fac = 1 / (len(w_num) ** dims)
+    et_n, ei_n = snapshot_energies(w_num, gamma)
+    et_t, ei_t = snapshot_energies(w_theo, gamma)
+    a = np.concatenate((w_num, et_n[..., None], ei_n[..., None]), axis=-1)
+    b = np.concatenate((w_theo, et_t[..., None], ei_t[..., None]), axis=-1)
+    ax = tuple(range(dims))
+    if p > 10:
+        return np.max(np.abs(a - b), axis=ax)
+    if p <= 0:
+        return fac * np.sum(np.abs(a - b), axis=ax)
+    return (fac * np.sum(np.abs(a - b) ** p, axis=ax)) ** (1 / p)
+
+
+def total_variation(w, gamma):
+    """analytic.py:48-62 for one snapshot: the successive differences along
+    every axis, summed in absolute value (8 primitives, E_tot, E_int)."""
+    dims = w.ndim - 1
+    et, ei = snapshot_energies(w, gamma)
+    g = np.copy(w)
+    for i in range(dims):
+        g, et, ei = np.diff(g, axis=i), np.diff(et, axis=i), np.diff(ei, axis=i)
+    tv = np.sum(np.abs(g), axis=tuple(range(dims)))
+    return np.append(np.append(tv, np.sum(np.abs(et))), np.sum(np.abs(ei)))
+
+
+def conservation(w, s, box_width=1.0):
+    """analytic.py:66-75 for one snapshot: convert_primitive (4th-order for
+    PPM/WENO, along the snapshot's own axes), summed, times box_width^dim."""
+    q = to_cons(w, s)
+    return np.sum(q, axis=tuple(range(w.ndim - 1))) * box_width ** (w.ndim - 1)

@@ -825,3 +825,43 @@ def test_snapshot_equals_reference_conversion(case):
         ref = ref.transpose(1, 0, 2)
     assert snap.shape == ref.shape
     assert np.array_equal(snap, ref)
+
+
+def test_diagnostics_of_device_snapshots():
+    """Total variation, conservation and solution error computed on the
+    device from device snapshots of a periodic shear layer, against the
+    oracle's restatements of analytic.py on the same state (rounding-level
+    agreement: the reductions sum in another order).  The periodic run keeps
+    the totals of its state; the snapshot-based conservation diagnostic of the
+    reference goes through the 4th-order primitive round trip and drifts by
+    O(h^4) except in the density."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import diagnostics as D, ic
+    import torch
+    n = 32
+    w0 = ic.shear_layer(n, 9)
+    q0 = ic.initial_average(torch.from_numpy(w0), 1.4, "wrap", True).numpy()
+    _say(f"[gpu] diagnostics of device snapshots, shear {n}^2")
+    plan = Plan(2, "ppm", "hllc", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=1.4, cfl=.4,
+                                                       t_end=1e9, max_steps=4)], flattener="astrea", varmask=0x17,
+                compat="physical")
+    plan.load([q0])
+    plan.reset()
+    c0 = D.conservation(plan.snapshot(0), 1.4, "wrap", True)
+    tot0 = plan.state.sum(dim=1).cpu().numpy()
+    plan.run(chunk=4)
+    snap = plan.snapshot(0)
+    state = plan.unpack(0)
+    tot1 = plan.state.sum(dim=1).cpu().numpy()
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid="ppm", boundary="wrap", gamma=1.4)
+    ref = O.to_prim(state, s).transpose(1, 0, 2)
+    rel = lambda a, b: np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+    assert rel(D.total_variation(snap, 1.4), O.total_variation(ref, 1.4)) <= 1e-12
+    c1 = D.conservation(snap, 1.4, "wrap", True)
+    assert rel(c1, O.conservation(ref, s)) <= 1e-12
+    theo = w0.transpose(1, 0, 2)
+    for p in (0, 2, 11):
+        assert rel(D.solution_error(snap, theo, 1.4, p), O.solution_error(ref, theo, 1.4, p)) <= 1e-12
+    assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.maximum(np.abs(tot0), 1))
+    assert abs(c1[0] - c0[0]) <= 1e-13 * c0[0]
