@@ -15,7 +15,7 @@ AH_EDGE, AH_WRAP = 0, 1
 AH_PCM, AH_PLM, AH_PPM, AH_WENO3, AH_WENO5, AH_WENO7 = range(6)
 AH_MINMOD, AH_VANLEER, AH_OSPRE, AH_VANALBADA1, AH_KOREN, AH_SUPERBEE = range(6)
 AH_FLAT_OFF, AH_FLAT_ASTREA, AH_FLAT_COLELLA90 = range(3)
-AH_LLF, AH_HLLC, AH_HLLD, AH_DOTS, AH_ES = range(5)
+AH_LLF, AH_HLLC, AH_HLLD, AH_DOTS, AH_ES, AH_LW = range(6)
 AH_EULER, AH_RK4, AH_SSPRK22, AH_SSPRK33, AH_SSPRK43, AH_SSPRK53, AH_SSPRK54, AH_SSPRK104 = range(8)
 AH_FP64, AH_SPLIT = 0, 1
 AH_COMPAT_ASTREA, AH_COMPAT_PHYSICAL = 0, 1

@@ -18,7 +18,7 @@ namespace ah {
 
 enum { EDGE = 0, WRAP = 1 };
 enum { PCM = 0, PLM = 1, PPM = 2, WENO3 = 3, WENO5 = 4, WENO7 = 5 };
-enum { LLF = 0, HLLC = 1, HLLD = 2, DOTS = 3, ES = 4 };
+enum { LLF = 0, HLLC = 1, HLLD = 2, DOTS = 3, ES = 4, LW = 5 };
 enum { MINMOD = 0, VANLEER = 1, OSPRE = 2, VANALBADA1 = 3, KOREN = 4, SUPERBEE = 5 };
 enum { FLAT_OFF = 0, FLAT_ASTREA = 1, FLAT_COLELLA90 = 2 };
 
@@ -421,6 +421,16 @@ __device__ inline V8<R> flux_llf(const Iface<R>& s, R lam) {
     for (int i = 0; i < 8; ++i)
         if (VS<MASK>::live(i)) F.c[i] = R(0.5) * (s.fm.c[i] + s.fp.c[i]) - R(0.5) * ((s.qp.c[i] - s.qm.c[i]) * lam);
     return F;
+}
+
+// Lax-Wendroff dissipation coefficient at a Jacobian state (solvers.py:82-85):
+// ss^2 / max|lambda| with ss the sound speed sqrt(gamma p / rho), the wave
+// speed the reference's comment names, and r = max|lambda| (the spectral
+// radius).  Paired (k, k+1) and applied as the LLF flux applies its radii.
+template <class R>
+__device__ inline R lw_coef(const V8<R>& w, R r, R gamma) {
+    const R ss = ah_sqrt(gamma * w.c[4] / w.c[0]);
+    return gdiv(ss * ss, r);
 }
 
 // HLLC (solvers.py:92-138).  sax = the solver's axis argument (the

@@ -106,8 +106,10 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
         if (mask == 0x13) {
             if (solver == LLF) return by_recon<R, 0x13, LLF, 1, SPLIT>(recon, ks);
             if (solver == HLLC) return by_recon<R, 0x13, HLLC, 1, SPLIT>(recon, ks);
-            if constexpr (!SPLIT)
+            if constexpr (!SPLIT) {
                 if (solver == DOTS) return by_recon<R, 0x13, DOTS, 1, SPLIT>(recon, ks);
+                if (solver == LW) return by_recon<R, 0x13, LW, 1, SPLIT>(recon, ks);
+            }
         }
         if (mask == 0xFF) {
             if (solver == LLF) return by_recon<R, 0xFF, LLF, 1, SPLIT>(recon, ks);
@@ -118,14 +120,19 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
                 // entropy-stable: 1D only (as the reference), all 8 components, whose
                 // non-finite values feed the reference's failure on the next step
                 if (solver == ES) return by_recon<R, 0xFF, ES, 1, SPLIT>(recon, ks);
+                // Lax-Wendroff with the hydrodynamic sound speed (transverse
+                // velocities; the reference's own pick is hydrodynamic only)
+                if (solver == LW) return by_recon<R, 0xFF, LW, 1, SPLIT>(recon, ks);
             }
         }
     } else {
         if (mask == 0x17) {
             if (solver == LLF) return by_recon<R, 0x17, LLF, 2, SPLIT>(recon, ks);
             if (solver == HLLC) return by_recon<R, 0x17, HLLC, 2, SPLIT>(recon, ks);
-            if constexpr (!SPLIT)
+            if constexpr (!SPLIT) {
                 if (solver == DOTS) return by_recon<R, 0x17, DOTS, 2, SPLIT>(recon, ks);
+                if (solver == LW) return by_recon<R, 0x17, LW, 2, SPLIT>(recon, ks);
+            }
         }
         if (mask == 0x77) {
             if (solver == LLF) return by_recon<R, 0x77, LLF, 2, SPLIT>(recon, ks);

@@ -267,7 +267,7 @@ __device__ inline V8<double> jac_state(const V8<double>& wp, const V8<double>& w
 template <class R, int MASK, int SOLVER, int AX>
 __device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma, int mom, bool physical,
                                bool split = false) {
-    if constexpr (SOLVER == LLF)
+    if constexpr (SOLVER == LLF || SOLVER == LW)
         return flux_llf<R, MASK>(s, lam);
     else if constexpr (SOLVER == HLLC)
         return split ? flux_hllc_split<MASK>(s, sax, gamma, physical ? mom : 1, physical)
@@ -500,7 +500,8 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         R r;
         if constexpr (SPLIT) r = radius<MASK, 0, true>(st, gamma);
         else r = spectral_radius<R, MASK, 0>(st, gR);
-        rad[i] = r;
+        if constexpr (SOLVER == LW) rad[i] = lw_coef<R>(st, r, gR);
+        else rad[i] = r;
         if (j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= N) {
             const double rd = double(r);
             if (!(rd <= 1.7976931348623157e308)) bad = true;
@@ -782,10 +783,11 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     __shared__ R fm_s[NV][NTA][NKF];
     __shared__ R fp_s[NV][NTA][NKF];
     __shared__ R fa_s[NV][NTA][NKF];
-    // only LLF reads the radii of the halo positions (its lambda); the other
-    // solvers need the radius of the tile's own interfaces (dt reduction and
-    // the non-finite check), which stage B then evaluates alone
-    constexpr bool LAM = SOLVER == LLF;
+    // only LLF and Lax-Wendroff read coefficients of the halo positions (the
+    // radii, or ss^2 / radius); the other solvers need the radius of the
+    // tile's own interfaces (dt reduction and the non-finite check), which
+    // stage B then evaluates alone
+    constexpr bool LAM = SOLVER == LLF || SOLVER == LW;
     __shared__ R rad_s[LAM ? NTA : 1][LAM ? NK : 1];
     __shared__ R bn_s[SOLVER == HLLD ? NTA : 1][SOLVER == HLLD ? NKF : 1];
     // When every thread owns at most one interface of stage D, the corrected
@@ -910,7 +912,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
         R r;
         if constexpr (SPLIT) r = radius<MASK, AX, true>(st, pd.gamma);
         else r = spectral_radius<R, MASK, AX>(st, gR);
-        if constexpr (LAM) rad_s[ia][ik] = r;
+        if constexpr (SOLVER == LW) rad_s[ia][ik] = lw_coef<R>(st, r, gR);
+        else if constexpr (LAM) rad_s[ia][ik] = r;
         if (t >= t0 && t < t0 + TT && t < nt && j >= s0 + 1 && j <= s0 + TS && j >= 1 && j <= ns) {
             const double rd = double(r);
             if (!(rd <= 1.7976931348623157e308)) bad = true;

@@ -143,7 +143,10 @@ def solver_of(solver, category=None):
         # solvers.py:20-24: names starting with "o" select Osher-Solomon, the rest entropy-stable
         return L.AH_DOTS if s.startswith("o") else L.AH_ES
     if s.endswith("w"):
-        raise NotImplementedError("Lax-Wendroff has no device path (its wave speed depends on LAPACK ordering)")
+        # solvers.py:27-28.  The device takes the sound speed sqrt(gamma p / rho)
+        # as the wave speed (solvers.py:83's comment); the reference's column
+        # pick from LAPACK's eigenvalue order is not reproduced.
+        return L.AH_LW
     return L.AH_LLF
 
 

@@ -52,8 +52,12 @@ enum ah_limiter { AH_MINMOD = 0, AH_VANLEER = 1, AH_OSPRE = 2, AH_VANALBADA1 = 3
 enum ah_flattener { AH_FLAT_OFF = 0, AH_FLAT_ASTREA = 1, AH_FLAT_COLELLA90 = 2 };
 /* solvers.py:69-75 (LLF), 92-138 (HLLC), 142-232 (HLLD), 236-284 (Osher-Solomon
    with the Dumbser-Toro path integral; FP64, 1D hydro/MHD and 2D hydro),
-   288-391 (entropy-stable as the reference writes it; FP64, 1D, all 8 variables) */
-enum ah_solver { AH_LLF = 0, AH_HLLC = 1, AH_HLLD = 2, AH_DOTS = 3, AH_ES = 4 };
+   288-391 (entropy-stable as the reference writes it; FP64, 1D, all 8 variables),
+   79-88 (Lax-Wendroff, FP64, hydrodynamic 1D/2D: the coefficient ss^2/max|lambda|
+   with ss the sound speed sqrt(gamma p/rho); the reference takes ss from a
+   LAPACK-order-dependent eigenvalue column instead, so its results are not
+   reproduced) */
+enum ah_solver { AH_LLF = 0, AH_HLLC = 1, AH_HLLD = 2, AH_DOTS = 3, AH_ES = 4, AH_LW = 5 };
 enum ah_rk {
     AH_EULER = 0, AH_RK4 = 1, AH_SSPRK22 = 2, AH_SSPRK33 = 3, AH_SSPRK43 = 4,
     AH_SSPRK53 = 5, AH_SSPRK54 = 6, AH_SSPRK104 = 7

@@ -43,7 +43,9 @@ PARITY = {
     "spectral radius (closed)": "golden localmax (LAPACK) to 1e-13 rel",
     "llf/hllc/hlld fluxes": "golden schemes_1d.npz",
     "es flux": "golden schemes_1d.npz *_flux_es (NaN positions included)",
-    "lw flux": "parity unpinned (reference picks a LAPACK-order-dependent column)",
+    "lw flux (flux_lw, LAPACK pick)": "golden schemes_1d.npz (12 hydro cases)",
+    "lw flux (sound-speed form)": "structure pinned through flux_lw_speed + consistency; "
+                                  "the speed choice departs from the reference by design",
     "2D double solve + transverse correction": "golden space_2d.npz",
     "MHD constrained transport": "golden space_2d.npz + steps ot16_*",
     "SSPRK/RK stages": "golden steps rk_* + amplification polynomials",
@@ -536,16 +538,32 @@ def flux_llf(lmax, d):
     return .5 * (fm + fp) - .5 * ((qp - qm) * lam[..., None])
 
 
-def flux_lw(chars, d):
-    """solvers.py:79-88.  Needs the LAPACK eigenvalues (its 'sound speed' is
-    the second unique eigenvalue column, which depends on LAPACK ordering):
-    parity unpinned."""
+def flux_lw_speed(ss, lmax, d):
+    """solvers.py:82-88 for a per-entry wave speed ``ss`` and max|lambda|
+    ``lmax`` on the padded Jacobian states: ss^2 / max|lambda| (guarded),
+    paired (k, k+1) as LLF pairs its radii, as the dissipation coefficient."""
     qp, qm = d["qFs"]
     fp, fm = d["fluxFs"]
-    ss = np.unique(chars, axis=-1)[..., 1]
-    ne = gdiv(ss ** 2, np.max(np.abs(chars), axis=-1))
+    ne = gdiv(ss ** 2, lmax)
     mx = np.max([ne[:-1], ne[1:]], axis=0)
     return .5 * (fm + fp) - .5 * ((qp - qm) * mx[..., None])
+
+
+def lw_sound_speed(w_jac, gamma):
+    """The wave speed solvers.py:83 names ("sound speed for each cell"),
+    sqrt(gamma p / rho) at the Jacobian states.  The closed-form path uses it
+    in place of the reference's column pick (``flux_lw``), which cannot be
+    reproduced without LAPACK: parity with the reference is not claimed."""
+    return np.sqrt(gamma * w_jac[..., 4] / w_jac[..., 0])
+
+
+def flux_lw(chars, d):
+    """solvers.py:79-88 as written: the 'sound speed' is the second unique
+    eigenvalue column, np.unique(chars, axis=-1)[..., 1] -- whole columns of
+    the flattened LAPACK output sorted lexicographically, so the pick is a
+    global decision that depends on LAPACK's eigenvalue order at every entry
+    (the zero eigenvalue's column varies with the state)."""
+    return flux_lw_speed(np.unique(chars, axis=-1)[..., 1], np.max(np.abs(chars), axis=-1), d)
 
 
 def flux_hllc(ax, gamma, d, mom=1, default_minus=False):
@@ -884,7 +902,9 @@ def riemann(s, ax, lmax, chars, d):
             return flux_dots(ax, s.gamma, d)
         return flux_es(ax, s.gamma, d)
     if s.solver.endswith("w"):
-        return flux_lw(chars, d)
+        if chars is not None:
+            return flux_lw(chars, d)
+        return flux_lw_speed(lw_sound_speed(d["jac_state"], s.gamma), lmax, d)
     return flux_llf(lmax, d)
 
 
@@ -1007,7 +1027,7 @@ def evolve_space(grid, s):
         eigmax = np.max(np.maximum(lmax[:-1], lmax[1:]))
         Favg = riemann(s, enum_ax, lmax, chars, d)
         if s.dimension == 2:
-            hd = {"wS": d["wS"]}
+            hd = {"wS": d["wS"], "jac_state": d["jac_state"]}
             for key in ("wFs", "qFs", "fluxFs"):
                 a, b = d[key]
                 hd[key] = (ho_convert("avg", a, s.boundary, [1]), ho_convert("avg", b, s.boundary, [1]))

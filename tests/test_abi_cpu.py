@@ -59,8 +59,7 @@ def test_solver_routing():
     assert O.solver_of("hllc") == L.AH_HLLC and O.solver_of("c") == L.AH_HLLC
     assert O.solver_of("d") == L.AH_HLLD
     assert O.solver_of("lax-wendroff") == L.AH_LLF     # solvers.py:28: only names ending in 'w' reach LW
-    with pytest.raises(NotImplementedError):
-        O.solver_of("lw")
+    assert O.solver_of("lw") == L.AH_LW                # names ending in 'w' (solvers.py:27-28)
     # solvers.py:20-24: "complete" names starting with "o" are Osher-Solomon, the rest entropy-stable
     for name in ("os", "osher", "osher-solomon", "osher solomon"):
         assert O.solver_of(name) == L.AH_DOTS

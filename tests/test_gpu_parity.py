@@ -92,6 +92,64 @@ def test_interface_flux_1d(key, solver):
     assert abs(fl[0]["eigmax"] - float(g[key + "_eigmax"])) <= 1e-14 * float(g[key + "_eigmax"])
 
 
+@pytest.mark.parametrize("key", SCHEME_KEYS)
+def test_lax_wendroff_interface_flux_1d(key):
+    """Lax-Wendroff with the sound-speed coefficient: ah_evolve_space fluxes
+    equal the oracle's (closed-form radius, same expression order) to the
+    last bit on the reference's 1D scheme inputs."""
+    from astrea_amd.plan import Plan
+    g = load("schemes_1d.npz")
+    _, sg, bc = key.split("_")
+    grid = g[key + "_grid"]
+    N = grid.shape[0]
+    _say(f"[gpu] lax-wendroff flux1d {key}")
+    plan = Plan(1, sg, "lw", "euler", [dict(n0=N, boundary=bc, dx=1 / 32, gamma=1.4, cfl=.5, t_end=1.)],
+                varmask=0x13)
+    plan.load([grid])
+    fl = plan.evolve_space(0)
+    plan.close()
+    s = O.Scheme(dimension=1, subgrid=sg, solver="lw", boundary=bc, gamma=1.4, dx=1 / 32)
+    d = O.reconstruct(grid, s, 0)
+    lm, _ = O.local_max_eig(d["jac_state"], s, 0)
+    assert np.array_equal(fl[0]["flux"], O.riemann(s, 0, lm, None, d))
+    assert fl[0]["eigmax"] == float(np.max(np.maximum(lm[:-1], lm[1:])))
+
+
+@pytest.mark.parametrize("dims,subgrid,bc", [(1, "plm", "edge"), (1, "ppm", "wrap"), (1, "weno5", "edge"),
+                                             (1, "pcm", "wrap"), (2, "ppm", "wrap"), (2, "plm", "edge")])
+def test_lax_wendroff_trajectory_vs_oracle(dims, subgrid, bc):
+    """Lax-Wendroff runs (1D Riemann problem 64 cells, 2D quadrants 16^2,
+    SSPRK(3,3)) against the oracle's sound-speed form: same steps, bitwise."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    if dims == 1:
+        n, steps = 64, 12
+        w = ic.riemann_1d(n, (1., .3, 1.), (.25, .1, .2))
+        prob = dict(n0=n, boundary=bc, dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9, max_steps=steps)
+        vm = 0x13
+    else:
+        n, steps = 16, 4
+        w = ic.quadrants(n, [(1, .2, .1, 1), (.5, -.3, .2, .4), (.2, .1, -.5, .2), (.8, 0, 0, .9)])
+        prob = dict(n0=n, n1=n, boundary=bc, dx=1 / n, gamma=1.4, cfl=.3, t_end=1e9, max_steps=steps)
+        vm = 0x17
+    q0 = ic.initial_average(torch.from_numpy(w), 1.4, bc, subgrid in ("ppm", "weno5")).numpy()
+    _say(f"[gpu] lax-wendroff trajectory {dims}D {subgrid} {bc} n={n} steps={steps}")
+    plan = Plan(dims, subgrid, "lw", "ssprk(3,3)", [prob], varmask=vm)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=steps)[0]
+    out = plan.unpack(0)
+    plan.close()
+    sch = O.Scheme(dimension=dims, subgrid=subgrid, solver="lw", timestep="ssprk(3,3)", boundary=bc, gamma=1.4,
+                   cfl=prob["cfl"], dx=1 / n)
+    ref = O.run(q0, sch, 1e9, max_steps=steps)
+    assert st["steps"] == steps == ref.steps and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert np.array_equal(out, ref.grid)
+
+
 DOTS_KEYS = [f"{p}_{sg}_{bc}" for p in ("hd", "mhd") for sg in ("pcm", "plm", "ppm", "weno3", "weno5", "weno7")
              for bc in ("edge", "wrap")]
 

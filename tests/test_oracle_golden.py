@@ -199,7 +199,38 @@ def test_weno5_linear_weights_on_linear_data():
     np.testing.assert_allclose(wL[3:-3, 0], (2 * (x - .05) + 1)[3:-3], rtol=1e-12)
 
 
-@pytest.mark.parametrize("solver", ["llf", "hllc"])
+@pytest.mark.parametrize("key", [k for k in SCHEME_KEYS if k.startswith("hd_")])
+def test_lax_wendroff_reference_pick(key):
+    """flux_lw, the reference's column pick np.unique(chars, axis=-1)[..., 1]
+    on LAPACK eigenvalues, gives the reference's Lax-Wendroff fluxes to the
+    last bit.  flux_lw_speed, which the sound-speed form shares, is pinned
+    through it."""
+    g = load("schemes_1d.npz")
+    _, sg, bc = key.split("_")
+    s = O.Scheme(dimension=1, subgrid=sg, solver="lw", boundary=bc, gamma=1.4, dx=1 / 32, eig="lapack")
+    d = O.reconstruct(g[key + "_grid"], s, 0)
+    lm, chars = O.local_max_eig(d["jac_state"], s, 0)
+    assert np.array_equal(O.riemann(s, 0, lm, chars, d), g[key + "_flux_lw"])
+
+
+def test_lax_wendroff_speed_form_reduces_to_llf():
+    """With the wave speed set to max|lambda| the coefficient ss^2/max|lambda|
+    is max|lambda| itself, and the Lax-Wendroff form is the LLF flux: checks
+    the (k, k+1) pairing and the signs against flux_llf."""
+    g = load("schemes_1d.npz")
+    s = O.Scheme(dimension=1, subgrid="ppm", solver="lw", boundary="edge", gamma=1.4, dx=1 / 32)
+    d = O.reconstruct(g["hd_ppm_edge_grid"], s, 0)
+    lm, _ = O.local_max_eig(d["jac_state"], s, 0)
+    F_lw = O.flux_lw_speed(lm, lm, d)
+    F_llf = O.flux_llf(lm, d)
+    np.testing.assert_allclose(F_lw, F_llf, rtol=1e-15, atol=1e-15)
+    # the sound speed is below max|lambda| = |u| + c, so LW dissipates less than LLF
+    ss = O.lw_sound_speed(d["jac_state"], 1.4)
+    assert np.all(ss ** 2 / lm <= lm)
+    assert not np.allclose(O.riemann(s, 0, lm, None, d), F_llf)
+
+
+@pytest.mark.parametrize("solver", ["llf", "hllc", "lw"])
 def test_riemann_consistency(solver):
     """F(w, w) = f(w) for a constant state."""
     N = 16
