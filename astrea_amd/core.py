@@ -1,10 +1,14 @@
 """Drop-in for astrea's time loop on the device.
 
-``core_run(grid, sim_variables)`` replaces astrea.py:31-85 (without its HDF5
-snapshot and plotting I/O): the initial conservative grid (the output of
+``core_run(grid, sim_variables)`` replaces astrea.py:31-85 (without its
+plotting): the initial conservative grid (the output of
 constructor.initialise) is advanced to ``t_end`` (or ``max_steps``) by the
-device plan, which keeps t, dt and the termination test on the GPU.
+device plan, which keeps t, dt and the termination test on the GPU.  With a
+``save`` sink it also hands over the snapshot of every pass of the loop, as
+the reference writes one HDF5 dataset per pass (astrea.py:46-50).
 """
+import torch
+
 from .plan import Plan, infer_varmask
 
 
@@ -21,12 +25,37 @@ def plan_for(sim_variables, grids, precision=None):
                 precision=precision or sv.precision_mode, varmask=mask, magnetic_2d=sv.magnetic_2d)
 
 
-def core_run(grid, sim_variables, precision=None, chunk=64):
-    """Returns (final grid in the reference layout, status dict)."""
+def run_saving(plan, save, t_end, i=0):
+    """Step problem ``i`` of ``plan`` one step per pass and call
+    ``save(str(float(t)), snapshot)`` on every pass with t <= t_end, before
+    the stop test (astrea.py:44-50): the initial state, each step's state, and
+    the state at t_end when the run ends there exactly.  A step that fails
+    (non-finite state) ends the loop without a snapshot, where the reference
+    raises.  The snapshot is ``Plan.snapshot`` copied to pinned host memory."""
+    dev = plan.snapshot(i)
+    host = torch.empty(tuple(dev.shape), dtype=torch.float64, pin_memory=True)
+    while True:
+        st = plan.status()[i]
+        if st["failed"]:
+            return st
+        if st["t"] <= t_end:
+            plan.snapshot(i, out=dev)
+            host.copy_(dev)
+            save(str(float(st["t"])), host.numpy().copy())
+        if not st["active"]:
+            return st
+        plan.step(1)
+
+
+def core_run(grid, sim_variables, precision=None, chunk=64, save=None):
+    """Returns (final grid in the reference layout, status dict).  ``save``
+    (e.g. ``output.SnapshotFile``) receives (str(float(t)), snapshot) on every
+    pass of the loop; without it the plan steps ``chunk`` steps between looks
+    at the device time state."""
     plan = plan_for(sim_variables, [grid], precision)
     plan.load([grid])
     plan.reset()
-    st = plan.run(chunk)
+    st = plan.run(chunk)[0] if save is None else run_saving(plan, save, sim_variables.t_end)
     out = plan.unpack(0)
     plan.close()
-    return out, st[0]
+    return out, st

@@ -1153,13 +1153,24 @@ class RunResult:
     dts: list = field(default_factory=list)
 
 
-def run(grid, s, t_end, checkpoints=1, max_steps=None):
-    """astrea.py:31-85 time loop without I/O: dt from the stage-0 eigmax,
-    clamp to the next checkpoint, exact-equality stop, sweep order flip."""
+def snapshot(grid, s):
+    """astrea.py:47: convert_conservative of the grid, transposed by
+    ortho_axis (generic.py:264: the identity in 1D, (1, 0, 2) in 2D)."""
+    w = to_prim(grid, s)
+    return w.transpose(1, 0, 2) if s.dimension == 2 else w
+
+
+def run(grid, s, t_end, checkpoints=1, max_steps=None, snapshots=None):
+    """astrea.py:31-85 time loop: dt from the stage-0 eigmax, clamp to the
+    next checkpoint, exact-equality stop, sweep order flip.  With a list
+    ``snapshots``, each pass of the loop appends (str(float(t)), snapshot)
+    before the stop test, as astrea.py:46-50 writes one HDF5 dataset per pass."""
     chkpt = t_end / checkpoints
     t, idx, dts, n = 0., 1, [], 0
     s = replace(s, parity=0)
     while t <= t_end:
+        if snapshots is not None:
+            snapshots.append((str(float(t)), snapshot(grid, s)))
         if t == t_end or (max_steps is not None and n >= max_steps):
             break
         fl = evolve_space(grid, s)

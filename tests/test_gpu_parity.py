@@ -885,6 +885,44 @@ def test_snapshot_equals_reference_conversion(case):
     assert np.array_equal(snap, ref)
 
 
+@pytest.mark.parametrize("case", ["sod_plm", "ll3_ppm"])
+def test_snapshot_series_equals_reference_loop(case, tmp_path):
+    """core_run with a SnapshotFile: one snapshot per pass of the time loop
+    (t = 0, every step, t_end), keys str(float(t)), against the oracle's
+    restatement of astrea.py:44-50 on the same initial state, bit for bit."""
+    from astrea_amd import core, ic, options
+    from astrea_amd.output import SnapshotFile
+    import torch
+    name, sub = case.split("_")
+    _say(f"[gpu] snapshot series {case}")
+    if name == "sod":
+        n, t_end, chk = 64, .05, 4
+        w = ic.sod(n)
+        sv = options.handle_variables({"config": "sod", "cells": n, "subgrid": sub, "solver": "hllc",
+                                       "timestep": "ssprk(3,3)", "cfl": .5, "checkpoints": chk}, "edge", t_end=t_end)
+    else:
+        n, t_end, chk = 16, .1, 3
+        w = ic.quadrants(n, [(1, .2, .1, 1), (.5, -.3, .2, .4), (.2, .1, -.5, .2), (.8, 0, 0, .9)])
+        sv = options.handle_variables({"config": "ll3", "cells": n, "subgrid": sub, "solver": "hllc",
+                                       "timestep": "ssprk(3,3)", "cfl": .4, "checkpoints": chk}, "edge", t_end=t_end)
+    assert sv.dimension == (1 if name == "sod" else 2)
+    q0 = ic.initial_average(torch.from_numpy(w), 1.4, "edge", sub == "ppm").numpy()
+    path = tmp_path / "series.npz"
+    with SnapshotFile(path) as f:
+        _, st = core.core_run(q0, sv, save=f)
+    s = O.Scheme(dimension=sv.dimension, subgrid=sub, solver="hllc", timestep="ssprk(3,3)", boundary="edge",
+                 gamma=1.4, cfl=sv.cfl, dx=sv.dx)
+    ref = []
+    r = O.run(q0, s, t_end, chk, snapshots=ref)
+    assert st["steps"] == r.steps and not st["failed"]
+    with np.load(path) as z:
+        keys = sorted(z.files, key=float)
+        _say(f"[gpu]   {len(keys)} snapshots, last t={keys[-1]}")
+        assert keys == [k for k, _ in ref]
+        for k, a in ref:
+            assert np.array_equal(z[k], a), k
+
+
 def test_diagnostics_of_device_snapshots():
     """Total variation, conservation and solution error computed on the
     device from device snapshots of a periodic shear layer, against the
