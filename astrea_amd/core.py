@@ -7,8 +7,13 @@ device plan, which keeps t, dt and the termination test on the GPU.  With a
 ``save`` sink it also hands over the snapshot of every pass of the loop, as
 the reference writes one HDF5 dataset per pass (astrea.py:46-50).
 """
+import os
+from datetime import datetime
+from time import perf_counter
+
 import torch
 
+from .output import SnapshotFile
 from .plan import Plan, infer_varmask
 
 
@@ -47,15 +52,74 @@ def run_saving(plan, save, t_end, i=0):
         plan.step(1)
 
 
-def core_run(grid, sim_variables, precision=None, chunk=64, save=None):
+def run_captured(plan, save, t_end, every="step", chunk=16, i=0):
+    """The series of ``run_saving`` (every="step") or its subset at t = 0 and
+    the checkpoints (every="checkpoint"), with the snapshots taken by the
+    step program itself (``Plan.capture``): the host looks at the device once
+    per chunk of steps, copies the chunk's snapshots to pinned memory ahead of
+    the next chunk on the stream, and calls ``save`` while that chunk runs."""
+    st = plan.status()[i]
+    if st["t"] <= t_end and not st["failed"]:
+        save(str(float(st["t"])), plan.snapshot(i).cpu().numpy())
+    plan.capture(i, every, slots=chunk)
+    host = torch.empty(tuple(plan._cap["ring"].shape), dtype=torch.float64, pin_memory=True)
+    stream = torch.cuda.current_stream(plan.device)
+    try:
+        plan.step(chunk)
+        while True:
+            st = plan.status()[i]
+            got = plan.drain()
+            for k, (_, snap) in enumerate(got):
+                host[k].copy_(snap, non_blocking=True)
+            copied = torch.cuda.Event()
+            copied.record(stream)
+            if st["active"]:
+                plan.step(chunk)
+            copied.synchronize()
+            for k, (t, _) in enumerate(got):
+                save(str(float(t)), host[k].numpy().copy())
+            if not st["active"]:
+                return st
+    finally:
+        plan.capture(i, None)
+
+
+def core_run(grid, sim_variables, precision=None, chunk=64, save=None, every="step"):
     """Returns (final grid in the reference layout, status dict).  ``save``
-    (e.g. ``output.SnapshotFile``) receives (str(float(t)), snapshot) on every
-    pass of the loop; without it the plan steps ``chunk`` steps between looks
-    at the device time state."""
+    (e.g. ``output.SnapshotFile``) receives (str(float(t)), snapshot) for
+    t = 0 and every step (every="step": the reference's series, astrea.py:46-50)
+    or every checkpoint (every="checkpoint"); the step program takes the
+    snapshots (``run_captured``).  Without ``save`` the plan steps ``chunk``
+    steps between looks at the device time state."""
     plan = plan_for(sim_variables, [grid], precision)
     plan.load([grid])
     plan.reset()
-    st = plan.run(chunk)[0] if save is None else run_saving(plan, save, sim_variables.t_end)
+    if save is None:
+        st = plan.run(chunk)[0]
+    else:
+        st = run_captured(plan, save, sim_variables.t_end, every, min(chunk, 16))
     out = plan.unpack(0)
     plan.close()
     return out, st
+
+
+def save_run(path, grid, sim_variables, seed=None, every="step", precision=None):
+    """One run of astrea.py:150-186 into a snapshot file: the file attributes
+    datetime ('%Y%m%d%H%M' of the first run) and seed, a group named by the
+    run's access key ('%Y%m%d%H%M%S' + microseconds) holding the run's
+    snapshot series and the attributes config, cells, cfl, gamma, dimension,
+    subgrid, timestep, solver and elapsed (seconds of core_run).  Appends to
+    an existing file.  Returns (access key, final grid, status)."""
+    sv = sim_variables
+    now = datetime.now()
+    key = now.strftime('%Y%m%d%H%M%S') + str(now.microsecond)
+    new = not os.path.exists(path)
+    root = {"datetime": now.strftime('%Y%m%d%H%M'), "seed": seed} if new else None
+    with SnapshotFile(path, "w" if new else "a", attrs=root) as f:
+        lap = perf_counter()
+        out, st = core_run(grid, sv, precision=precision, save=lambda k, a: f(f"{key}/{k}", a), every=every)
+        elapsed = perf_counter() - lap
+        f.set_attrs(key, {"config": sv.config, "cells": int(sv.cells), "cfl": float(sv.cfl),
+                          "gamma": float(sv.gamma), "dimension": int(sv.dimension), "subgrid": sv.subgrid,
+                          "timestep": sv.timestep, "solver": sv.solver, "elapsed": elapsed})
+    return key, out, st

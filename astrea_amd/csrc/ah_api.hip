@@ -89,6 +89,8 @@ struct ah_plan {
     // captured step graphs (ah_plan_step_graph)
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
+    // snapshots taken inside the step program (ah_plan_set_capture); ctl null: off
+    CaptureArgs cap = {};
     // bound memory
     double* U = nullptr;
     double* regs[9] = {nullptr};
@@ -150,7 +152,7 @@ struct ProfScope {
 
 namespace {
 template <int MASK>
-void launch_snapshot(const ah_plan* p, int prob, double* out, hipStream_t s) {
+void launch_snapshot(const ah_plan* p, int prob, double* out, hipStream_t s, const CaptureArgs& cap) {
     const ah_problem& q = p->probs[prob];
     const int n1 = p->sch.dims == 1 ? 1 : q.n1;
     const int bc = q.boundary == AH_WRAP ? WRAP : EDGE;
@@ -160,12 +162,22 @@ void launch_snapshot(const ah_plan* p, int prob, double* out, hipStream_t s) {
     const dim3 g((unsigned)((cells + 255) / 256)), b(256);
     const long long off = p->offs[prob];
     if (p->sch.dims == 1) {
-        if (ho) hipLaunchKernelGGL((snapshot_kernel<MASK, 1, true>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
-        else hipLaunchKernelGGL((snapshot_kernel<MASK, 1, false>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
+        if (ho) hipLaunchKernelGGL((snapshot_kernel<MASK, 1, true>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out, cap);
+        else hipLaunchKernelGGL((snapshot_kernel<MASK, 1, false>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out, cap);
     } else {
-        if (ho) hipLaunchKernelGGL((snapshot_kernel<MASK, 2, true>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
-        else hipLaunchKernelGGL((snapshot_kernel<MASK, 2, false>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out);
+        if (ho) hipLaunchKernelGGL((snapshot_kernel<MASK, 2, true>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out, cap);
+        else hipLaunchKernelGGL((snapshot_kernel<MASK, 2, false>), g, b, 0, s, p->U, p->total, off, q.n0, n1, bc, q.gamma, out, cap);
     }
+}
+
+bool snapshot_any(const ah_plan* p, int prob, double* out, hipStream_t s, const CaptureArgs& cap) {
+    switch (p->sch.varmask) {
+        case AH_VARS_H1: launch_snapshot<AH_VARS_H1>(p, prob, out, s, cap); return true;
+        case AH_VARS_H2: launch_snapshot<AH_VARS_H2>(p, prob, out, s, cap); return true;
+        case AH_VARS_M6: launch_snapshot<AH_VARS_M6>(p, prob, out, s, cap); return true;
+        case AH_VARS_ALL: launch_snapshot<AH_VARS_ALL>(p, prob, out, s, cap); return true;
+    }
+    return false;
 }
 }  // namespace
 
@@ -561,8 +573,14 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
         }
     }
-    ProfScope ps_(p, AH_K_ADVANCE, s);
-    hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+    {
+        ProfScope ps_(p, AH_K_ADVANCE, s);
+        hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+    }
+    if (p->cap.ctl) {
+        snapshot_any(p, p->cap.prob, nullptr, s, p->cap);
+        hipLaunchKernelGGL(capture_commit_kernel, dim3(1), dim3(64), 0, s, p->cap);
+    }
 }
 
 int ah_plan_step(ah_plan* p, int nsteps, void* stream) {
@@ -635,14 +653,38 @@ int ah_plan_init_ic(ah_plan* p, int prob, const ah_ic* ic, void* stream) {
 int ah_plan_snapshot(ah_plan* p, int prob, double* out, void* stream) {
     if (!p || !out || prob < 0 || prob >= p->nprob) return AH_ERR_INVALID;
     if (!p->bound) return AH_ERR_STATE;
-    hipStream_t s = (hipStream_t)stream;
-    switch (p->sch.varmask) {
-        case AH_VARS_H1: launch_snapshot<AH_VARS_H1>(p, prob, out, s); break;
-        case AH_VARS_H2: launch_snapshot<AH_VARS_H2>(p, prob, out, s); break;
-        case AH_VARS_M6: launch_snapshot<AH_VARS_M6>(p, prob, out, s); break;
-        case AH_VARS_ALL: launch_snapshot<AH_VARS_ALL>(p, prob, out, s); break;
-        default: return AH_ERR_UNSUPPORTED;
+    CaptureArgs none;
+    memset(&none, 0, sizeof(none));
+    if (!snapshot_any(p, prob, out, (hipStream_t)stream, none)) return AH_ERR_UNSUPPORTED;
+    HIPCHK(hipGetLastError());
+    return AH_OK;
+}
+
+int ah_plan_set_capture(ah_plan* p, int prob, int mode, double* ring, int slots, double* times, long long* ctl,
+                        void* stream) {
+    if (!p || prob < 0 || prob >= p->nprob || mode < AH_CAPTURE_OFF || mode > AH_CAPTURE_CHECKPOINTS)
+        return AH_ERR_INVALID;
+    if (mode != AH_CAPTURE_OFF && (!ring || !times || !ctl || slots <= 0)) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    // a captured step graph holds the previous capture arguments
+    if (p->graph_exec) {
+        HIPCHK(hipGraphExecDestroy(p->graph_exec));
+        p->graph_exec = nullptr;
+        p->graph_steps = 0;
     }
+    memset(&p->cap, 0, sizeof(p->cap));
+    if (mode == AH_CAPTURE_OFF) return AH_OK;
+    const ah_problem& q = p->probs[prob];
+    p->cap.ps = p->d_ps;
+    p->cap.ring = ring;
+    p->cap.times = times;
+    p->cap.ctl = ctl;
+    p->cap.cells = (long long)q.n0 * (p->sch.dims == 1 ? 1 : q.n1);
+    p->cap.t_end = q.t_end;
+    p->cap.prob = prob;
+    p->cap.mode = mode;
+    p->cap.slots = slots;
+    hipLaunchKernelGGL(capture_init_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, p->cap);
     HIPCHK(hipGetLastError());
     return AH_OK;
 }

@@ -25,10 +25,13 @@ __global__ void dt_kernel(const ProbDesc* pd, ProbState* ps, int nprob, int dims
     }
     const double a0 = d.dx / e0, a1 = d.dx / e1;
     double dt = d.cfl * (dims == 2 ? (a1 < a0 ? a1 : a0) : a0);
+    int hit = 0;
     if (s.t + dt >= d.chkpt * s.idx) {
         dt = d.chkpt * s.idx - s.t;
         s.idx += 1;
+        hit = 1;
     }
+    s.ckpt = hit;
     s.dt = dt;
 }
 
@@ -65,7 +68,7 @@ __global__ void reset_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
     s.parity = 0;
     s.eig[0] = s.eig[1] = 0ull;
     s.fail_flag = 0;
-    s.pad = 0;
+    s.ckpt = 0;
     const ProbDesc d = pd[p];
     s.active = (0.0 <= d.t_end) && !(0.0 == d.t_end) && d.max_steps != 0;
     ps[p] = s;

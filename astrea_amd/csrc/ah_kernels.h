@@ -28,7 +28,8 @@ struct ProbState {
     long long steps;
     int idx, active, failed, parity;
     unsigned long long eig[2];  // bit pattern of max spectral radius per axis (stage 0)
-    int fail_flag, pad;
+    int fail_flag;
+    int ckpt;  // 1 when this step's dt was clamped to the next checkpoint (astrea.py:74-78)
 };
 
 struct Tile {

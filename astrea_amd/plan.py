@@ -38,6 +38,7 @@ class Plan:
         self.lib = L.load()
         self.dims = dims
         self.device = torch.device(device)
+        self._cap = None
         sch = L.AhScheme()
         sch.dims = dims
         sch.recon = O.recon_of(subgrid)
@@ -202,6 +203,44 @@ class Plan:
             raise ValueError(f"snapshot buffer must be a contiguous float64 tensor of shape {shape}")
         L.check(self.lib.ah_plan_snapshot(self.h, int(i), ctypes.c_void_p(out.data_ptr()), self._stream()),
                 "ah_plan_snapshot")
+        return out
+
+    def capture(self, i, every="step", slots=64):
+        """Let the step program take problem i's snapshots (ah_plan_set_capture):
+        after every step (every="step") or after the steps whose dt was
+        clamped to a checkpoint (every="checkpoint"), into a device ring of
+        `slots` snapshots; every=None stops it.  ``drain`` reads them."""
+        mode = {None: L.AH_CAPTURE_OFF, "step": L.AH_CAPTURE_EVERY_STEP,
+                "checkpoint": L.AH_CAPTURE_CHECKPOINTS}[every]
+        if mode == L.AH_CAPTURE_OFF:
+            L.check(self.lib.ah_plan_set_capture(self.h, int(i), mode, None, 0, None, None, self._stream()),
+                    "ah_plan_set_capture")
+            self._cap = None
+            return
+        if slots <= 0:
+            raise ValueError("slots must be positive")
+        shape = self.shape(i)[::-1] + (8,)
+        ring = torch.empty((slots,) + shape, dtype=torch.float64, device=self.device)
+        times = torch.empty(slots, dtype=torch.float64, device=self.device)
+        ctl = torch.empty(2, dtype=torch.int64, device=self.device)
+        L.check(self.lib.ah_plan_set_capture(self.h, int(i), mode, ctypes.c_void_p(ring.data_ptr()), int(slots),
+                                             ctypes.c_void_p(times.data_ptr()), ctypes.c_void_p(ctl.data_ptr()),
+                                             self._stream()), "ah_plan_set_capture")
+        self._cap = dict(ring=ring, times=times, ctl=ctl, slots=slots, read=0)
+
+    def drain(self):
+        """The snapshots captured since the last drain, oldest first, as
+        (t, ring slot view) pairs; a slot is overwritten once the step
+        program has taken `slots` more snapshots.  Synchronises the stream."""
+        c = self._cap
+        if c is None:
+            raise L.AstreaHipError("drain: no capture set (Plan.capture)")
+        n = int(c["ctl"][0].item())
+        if n - c["read"] > c["slots"]:
+            raise L.AstreaHipError(f"capture ring overran: {n - c['read']} snapshots for {c['slots']} slots")
+        times = c["times"].cpu()
+        out = [(float(times[k % c["slots"]]), c["ring"][k % c["slots"]]) for k in range(c["read"], n)]
+        c["read"] = n
         return out
 
     def bytes_per_cell(self):

@@ -203,6 +203,23 @@ int ah_plan_init_ic(ah_plan* plan, int prob, const ah_ic* ic, void* stream);
  * stays with the caller. */
 int ah_plan_snapshot(ah_plan* plan, int prob, double* out, void* stream);
 
+/* Snapshots taken by the step program itself (astrea.py:46-50 writes one per
+ * pass of its loop; the host need not wait on the device between steps).
+ * After every step of problem `prob` (AH_CAPTURE_EVERY_STEP), or only after
+ * the steps whose dt was clamped to a checkpoint t_end*idx/checkpoints
+ * (AH_CAPTURE_CHECKPOINTS, astrea.py:74-78), the snapshot of
+ * ah_plan_snapshot goes to slot ctl[0] % slots of `ring` (device memory,
+ * slots * cells * 8 doubles), its t to times[slot] (device, `slots` doubles),
+ * and ctl[0] (device, 2 long longs; ctl[1] is internal) counts the snapshots
+ * taken. Steps that fail or do not happen take none, nor does a step that
+ * ends past t_end (the reference's loop stops without saving there). The caller drains the
+ * ring before ctl[0] runs `slots` ahead of what it has read. The launches are
+ * part of ah_plan_step and of captured step graphs; a call re-captures graphs
+ * and restarts the count. AH_CAPTURE_OFF (buffers may be null) stops it. */
+enum ah_capture { AH_CAPTURE_OFF = 0, AH_CAPTURE_EVERY_STEP = 1, AH_CAPTURE_CHECKPOINTS = 2 };
+int ah_plan_set_capture(ah_plan* plan, int prob, int mode, double* ring, int slots, double* times,
+                        long long* ctl, void* stream);
+
 /* Export / import the time state of all problems of a plan (synchronous),
  * so a problem can move between plans with its state planes: the serving
  * loop re-plans its batch as requests arrive and finish.  No reference

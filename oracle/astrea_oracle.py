@@ -1151,6 +1151,7 @@ class RunResult:
     t: float
     steps: int
     dts: list = field(default_factory=list)
+    clamped: list = field(default_factory=list)   # steps (1-based) whose dt was clamped to a checkpoint
 
 
 def snapshot(grid, s):
@@ -1167,6 +1168,7 @@ def run(grid, s, t_end, checkpoints=1, max_steps=None, snapshots=None):
     before the stop test, as astrea.py:46-50 writes one HDF5 dataset per pass."""
     chkpt = t_end / checkpoints
     t, idx, dts, n = 0., 1, [], 0
+    clamped = []
     s = replace(s, parity=0)
     while t <= t_end:
         if snapshots is not None:
@@ -1178,12 +1180,13 @@ def run(grid, s, t_end, checkpoints=1, max_steps=None, snapshots=None):
         if t + dt >= chkpt * idx:
             dt = chkpt * idx - t
             idx += 1
+            clamped.append(n + 1)
         grid = evolve_time(grid, fl, dt, s)
         t += dt
         dts.append(dt)
         n += 1
         s = replace(s, parity=1 - s.parity)
-    return RunResult(grid, t, n, dts)
+    return RunResult(grid, t, n, dts, clamped)
 
 
 def initial_average(w_point, s):

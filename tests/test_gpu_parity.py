@@ -885,42 +885,121 @@ def test_snapshot_equals_reference_conversion(case):
     assert np.array_equal(snap, ref)
 
 
-@pytest.mark.parametrize("case", ["sod_plm", "ll3_ppm"])
-def test_snapshot_series_equals_reference_loop(case, tmp_path):
-    """core_run with a SnapshotFile: one snapshot per pass of the time loop
-    (t = 0, every step, t_end), keys str(float(t)), against the oracle's
-    restatement of astrea.py:44-50 on the same initial state, bit for bit."""
-    from astrea_amd import core, ic, options
-    from astrea_amd.output import SnapshotFile
+def _series_case(case):
+    from astrea_amd import ic, options
     import torch
     name, sub = case.split("_")
-    _say(f"[gpu] snapshot series {case}")
     if name == "sod":
-        n, t_end, chk = 64, .05, 4
+        n, t_end, chk, cfl = 64, .05, 4, .5
         w = ic.sod(n)
-        sv = options.handle_variables({"config": "sod", "cells": n, "subgrid": sub, "solver": "hllc",
-                                       "timestep": "ssprk(3,3)", "cfl": .5, "checkpoints": chk}, "edge", t_end=t_end)
     else:
-        n, t_end, chk = 16, .1, 3
+        n, t_end, chk, cfl = 16, .1, 3, .4
         w = ic.quadrants(n, [(1, .2, .1, 1), (.5, -.3, .2, .4), (.2, .1, -.5, .2), (.8, 0, 0, .9)])
-        sv = options.handle_variables({"config": "ll3", "cells": n, "subgrid": sub, "solver": "hllc",
-                                       "timestep": "ssprk(3,3)", "cfl": .4, "checkpoints": chk}, "edge", t_end=t_end)
+    sv = options.handle_variables({"config": name, "cells": n, "subgrid": sub, "solver": "hllc",
+                                   "timestep": "ssprk(3,3)", "cfl": cfl, "checkpoints": chk}, "edge", t_end=t_end)
     assert sv.dimension == (1 if name == "sod" else 2)
     q0 = ic.initial_average(torch.from_numpy(w), 1.4, "edge", sub == "ppm").numpy()
-    path = tmp_path / "series.npz"
-    with SnapshotFile(path) as f:
-        _, st = core.core_run(q0, sv, save=f)
     s = O.Scheme(dimension=sv.dimension, subgrid=sub, solver="hllc", timestep="ssprk(3,3)", boundary="edge",
                  gamma=1.4, cfl=sv.cfl, dx=sv.dx)
     ref = []
     r = O.run(q0, s, t_end, chk, snapshots=ref)
+    return sv, q0, r, ref
+
+
+@pytest.mark.parametrize("every", ["step", "checkpoint"])
+@pytest.mark.parametrize("case", ["sod_plm", "ll3_ppm"])
+def test_snapshot_series_equals_reference_loop(case, every, tmp_path):
+    """core_run with a SnapshotFile, snapshots taken by the step program:
+    every="step" gives one snapshot per pass of the time loop (t = 0, every
+    step, t_end) keyed str(float(t)), equal to the oracle's restatement of
+    astrea.py:44-50 key for key and bit for bit; every="checkpoint" gives its
+    subset at t = 0 and after the steps whose dt was clamped to a checkpoint."""
+    from astrea_amd import core
+    from astrea_amd.output import SnapshotFile
+    _say(f"[gpu] snapshot series {case} every={every}")
+    sv, q0, r, ref = _series_case(case)
+    path = tmp_path / "series.npz"
+    with SnapshotFile(path) as f:
+        _, st = core.core_run(q0, sv, save=f, every=every)
     assert st["steps"] == r.steps and not st["failed"]
+    if every == "checkpoint":
+        assert r.clamped
+        ref = [ref[0]] + [ref[k] for k in r.clamped if k < len(ref)]
     with np.load(path) as z:
         keys = sorted(z.files, key=float)
         _say(f"[gpu]   {len(keys)} snapshots, last t={keys[-1]}")
         assert keys == [k for k, _ in ref]
         for k, a in ref:
             assert np.array_equal(z[k], a), k
+
+
+def test_save_run_layout(tmp_path):
+    """core.save_run: file attributes, a group per run with the run
+    attributes, the checkpoint series inside it equal to the oracle's."""
+    from astrea_amd import core
+    from astrea_amd.output import read_attrs
+    _say("[gpu] save_run sod_plm, two runs into one file")
+    sv, q0, r, ref = _series_case("sod_plm")
+    path = tmp_path / "runs.npz"
+    k1, _, st = core.save_run(path, q0, sv, seed=11, every="checkpoint")
+    k2, _, _ = core.save_run(path, q0, sv, seed=11, every="checkpoint")
+    assert k1 != k2 and st["steps"] == r.steps
+    assert read_attrs(path)["seed"] == 11
+    a = read_attrs(path, k1)
+    assert (a["config"], a["cells"], a["subgrid"], a["solver"], a["dimension"]) == ("sod", 64, "plm", "hllc", 1)
+    assert a["elapsed"] > 0
+    want = [ref[0]] + [ref[k] for k in r.clamped if k < len(ref)]
+    with np.load(path) as z:
+        for key in (k1, k2):
+            got = sorted((f for f in z.files if f.startswith(key + "/") and not f.endswith(".json")),
+                         key=lambda f: float(f.split("/")[1]))
+            assert [g.split("/")[1] for g in got] == [k for k, _ in want]
+            for g, (_, w) in zip(got, want):
+                assert np.array_equal(z[g], w)
+
+
+def test_snapshot_capture_host_loop_and_graph_agree():
+    """The same series three ways on a 2D problem: the host-stepped loop
+    (run_saving), the step program's capture with eager chunks, and the
+    capture inside replayed step graphs with a ring of 3 slots drained after
+    every 2-step replay (slot reuse)."""
+    from astrea_amd import core
+    from astrea_amd.core import plan_for
+    import torch
+    sv, q0, r, ref = _series_case("ll3_ppm")
+    _say(f"[gpu] capture host loop / eager / graph, {r.steps} steps")
+    got = {}
+    for how in ("host", "eager", "graph"):
+        plan = plan_for(sv, [q0])
+        plan.load([q0])
+        plan.reset()
+        series = []
+        save = lambda k, a: series.append((k, a))
+        if how == "host":
+            core.run_saving(plan, save, sv.t_end)
+        elif how == "eager":
+            core.run_captured(plan, save, sv.t_end, "step", chunk=4)
+        else:
+            side = torch.cuda.Stream(plan.device)
+            side.wait_stream(torch.cuda.current_stream(plan.device))
+            with torch.cuda.stream(side):
+                save("0.0", plan.snapshot(0).cpu().numpy())
+                plan.capture(0, "step", slots=3)
+                while True:
+                    plan.step_graph(2)
+                    st = plan.status()[0]
+                    for t, a in plan.drain():
+                        save(str(float(t)), a.cpu().numpy())
+                    if not st["active"]:
+                        break
+                plan.capture(0, None)
+            torch.cuda.current_stream(plan.device).wait_stream(side)
+        plan.close()
+        got[how] = series
+    for how, series in got.items():
+        assert [k for k, _ in series] == [k for k, _ in ref], how
+        for (k, a), (_, b) in zip(series, ref):
+            assert np.array_equal(a, b), (how, k)
 
 
 def test_diagnostics_of_device_snapshots():

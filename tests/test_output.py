@@ -81,3 +81,25 @@ def test_saving_loop_overshoot_and_failure(monkeypatch):
     # a step limit stops after the snapshot of the last state
     got, st = _saved(_ScriptedPlan([0., .1, .2, .3], 1., max_steps=2), 1., monkeypatch)
     assert [k for k, _ in got] == ["0.0", "0.1", "0.2"] and st["steps"] == 2
+
+
+def test_snapshot_file_groups_and_attributes(tmp_path):
+    """The reference's layout (astrea.py:150-186): file attributes, one group
+    per run with its attributes and its snapshot series."""
+    from astrea_amd.output import read_attrs
+    path = tmp_path / "runs.npz"
+    with SnapshotFile(path, attrs={"datetime": "202612021200", "seed": 7}) as f:
+        f("20261202120000123/0.0", np.ones((4, 8)))
+        f("20261202120000123/0.5", np.zeros((4, 8)))
+        f.set_attrs("20261202120000123", {"config": "sod", "cells": 4, "elapsed": 1.5})
+        with pytest.raises(ValueError):
+            f.set_attrs("20261202120000123", {})
+        assert f.keys("20261202120000123") == ["0.0", "0.5"] and f.keys() == []
+    with SnapshotFile(path, "a") as f:
+        f("20261202120001456/0.0", np.ones((4, 8)))
+        f.set_attrs("20261202120001456", {"config": "sod", "cells": 4, "elapsed": 2.})
+    assert read_attrs(path) == {"datetime": "202612021200", "seed": 7}
+    assert read_attrs(path, "20261202120000123")["elapsed"] == 1.5
+    with np.load(path) as z:
+        assert np.array_equal(z["20261202120000123/0.5"], np.zeros((4, 8)))
+        assert sum(k.endswith("/0.0") for k in z.files) == 2
