@@ -13,6 +13,7 @@ from time import perf_counter
 
 import torch
 
+from . import ic, options
 from .output import SnapshotFile
 from .plan import Plan, infer_varmask
 
@@ -123,3 +124,18 @@ def save_run(path, grid, sim_variables, seed=None, every="step", precision=None)
                           "gamma": float(sv.gamma), "dimension": int(sv.dimension), "subgrid": sv.subgrid,
                           "timestep": sv.timestep, "solver": sv.solver, "elapsed": elapsed})
     return key, out, st
+
+
+def simulate(params, path=None, seed=None, every="step", precision=None):
+    """A single run of astrea.py:run() for a parameters mapping: the options
+    and the configuration's test conditions (``options.handle_variables``),
+    the initial state of constructor.initialise (``ic.initialise``), the
+    device time loop, and with ``path`` the snapshot file of ``save_run``.
+    Returns (final grid, status, sim_variables)."""
+    sv = options.handle_variables(params)
+    grid = ic.initialise(sv)
+    if path is None:
+        out, st = core_run(grid, sv, precision=precision)
+    else:
+        _, out, st = save_run(path, grid, sv, seed, every, precision)
+    return out, st, sv

@@ -166,6 +166,81 @@ def quadrant_requests(nreq, seed, sizes=(128, 256, 512, 1024)):
     return out
 
 
+def initial_point(sv):
+    """constructor.py:11-103: the pointwise primitive state of configuration
+    sv.config (tables of ``configs.test_conditions`` in sv) on sv.cells cells
+    per axis, in numpy FP64 with the reference's operations in its order."""
+    config, N, gamma, dim = sv.config, sv.cells, sv.gamma, sv.dimension
+    lo, hi, x0, prm = sv.start_pos, sv.end_pos, sv.shock_pos, sv.misc
+    left = np.asarray(sv.initial_left)
+    grid = np.zeros((N,) * dim + (len(sv.initial_right),))
+    grid[:] = sv.initial_right
+    h = abs(hi - lo) / N
+    xs = np.linspace(lo - h / 2, hi + h / 2, N + 2)[1:-1]
+    if dim == 2:
+        x, y = np.meshgrid(xs, xs, indexing="ij")
+        c = (hi + lo) / 2
+        if config == "sedov" or "blast" in config:
+            grid[((x - c) ** 2 + (y - c) ** 2) <= (x0 - c) ** 2] = left
+        elif config.startswith("gauss"):
+            r = np.sqrt((x - c) ** 2 + (y - c) ** 2)
+            grid[..., 0] = prm["y_offset"] + prm["ampl"] * np.exp(-((r - c) ** 2) / prm["fwhm"])
+        elif config in ("khi", "kelvin-helmholtz") or "kelvin" in config or "helmholtz" in config:
+            grid[y <= x0] = left
+            grid[..., 2] = prm["perturb_ampl"] * np.sin(prm["freq"] * np.pi * x / (hi - lo))
+        elif config in ("ivc", "vortex", "isentropic vortex"):
+            xc, yc = (np.min(x) + np.max(x)) / 2, (np.min(y) + np.max(y)) / 2
+            r = np.sqrt((x - xc) ** 2 + (y - yc) ** 2)
+            k = prm["freq"] * np.pi
+            T = 1 - (((gamma - 1) * prm["vortex_str"] ** 2) / (2 * gamma * k ** 2)) * np.exp(1 - r ** 2)
+            grid[..., 0] = T ** (1 / (gamma - 1))
+            grid[..., 1] = (prm["vortex_str"] / k) * np.exp((1 - r ** 2) / 2)
+            grid[..., 2] = (prm["vortex_str"] / k) * np.exp((1 - r ** 2) / 2)
+            grid[..., 4] = T ** (gamma / (gamma - 1))
+        elif "ll" in config or "lax-liu" in config:
+            grid[x <= x0] = left
+            grid[(x <= x0) & (y >= x0)] = prm["bottom_left"]
+            grid[(x > x0) & (y >= x0)] = prm["bottom_right"]
+        elif config in ("orszag-tang", "orszag", "tang", "ot"):
+            grid[y <= x0] = left
+            grid[..., 1] = -np.sin(2 * np.pi * y)
+            grid[..., 2] = np.sin(2 * np.pi * x)
+            grid[..., 5] = -prm["ampl"] * np.sin(2 * np.pi * y)
+            grid[..., 6] = prm["ampl"] * np.sin(4 * np.pi * x)
+        elif "rotor" in config:
+            # constructor.py:79-81 writes the rotation into a fancy-indexed
+            # copy, so the rotor keeps zero velocity
+            grid[((x - c) ** 2 + (y - c) ** 2) <= (x0 - c) ** 2] = left
+        else:
+            grid[x < x0] = left
+    else:
+        x = xs
+        grid[(np.abs(x) <= x0) if (config == "sedov" or config.startswith("sq")) else (x <= x0)] = left
+        if "shu" in config or "osher" in config:
+            xr = x[x > x0]
+            grid[x > x0, 0] = prm["y_offset"] + prm["ampl"] * np.sin(prm["freq"] * np.pi * xr)
+        elif config.startswith("sin"):
+            grid[..., 0] = prm["y_offset"] + prm["ampl"] * np.sin(prm["freq"] * np.pi * x)
+        elif config.startswith("gauss"):
+            grid[..., 0] = prm["y_offset"] + prm["ampl"] * np.exp(-((x - prm["peak_pos"]) ** 2) / prm["fwhm"])
+        elif config.startswith("lin"):
+            if "mhd" in config:   # Alfven wave eigenvector (constructor.py:94-101)
+                pert = np.array([0, 0, -.3333333333333333, .9428090415820634, 0, -.3333333333333333,
+                                 .9428090415820634, 0]) * prm["ampl"]
+            else:
+                pert = np.array([1, -1, 1, 1, 1.5, 0, 0, 0]) * prm["ampl"]
+            grid += pert * np.sin(prm["freq"] * np.pi * x)[..., None]
+    return grid
+
+
+def initialise(sv):
+    """constructor.initialise(sv, convert=True) (constructor.py:11-109): the
+    averaged conservative initial state of sv.config, as numpy FP64."""
+    high = sv.subgrid.startswith("w") or sv.subgrid in ("ppm", "parabolic", "p")
+    w = torch.from_numpy(initial_point(sv))
+    return initial_average(w, sv.gamma, sv.boundary, high).numpy()
+
+
 def to_device(w_point, gamma, bc, high_order, device="cuda:0"):
     """Averaged conservative initial state (host pipeline), on ``device``."""
     w = torch.as_tensor(w_point, dtype=torch.float64, device="cpu")

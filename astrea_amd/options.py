@@ -12,6 +12,7 @@ import itertools
 from collections import namedtuple
 
 from . import _lib as L
+from . import configs
 
 # static/.db.json accepted values (the "type" -> "category" -> "accepted" table)
 ACCEPTED = {
@@ -60,7 +61,8 @@ DEFAULTS = {"config": "sod", "dimension": 1, "precision": "float64", "subgrid": 
 SimVariables = namedtuple("SimVariables", [
     "config", "cells", "cfl", "gamma", "dimension", "precision", "subgrid", "timestep", "solver", "run_type",
     "checkpoints", "limiter", "flattener", "precision_mode", "max_steps", "config_category", "solver_category",
-    "magnetic_2d", "permutations", "ortho_axis", "boundary", "dx", "t_end"])
+    "magnetic_2d", "permutations", "ortho_axis", "boundary", "dx", "t_end",
+    "start_pos", "end_pos", "shock_pos", "misc", "initial_left", "initial_right"])
 
 
 def _category(key, value):
@@ -70,10 +72,13 @@ def _category(key, value):
     return None
 
 
-def handle_variables(params, boundary="edge", dx=None, t_end=0.2):
-    """generic.py:159-286 for the keys the update reads.  ``params`` is the
-    flattened parameters.yml mapping; invalid values revert to the defaults
-    exactly as the reference does (with a printed warning)."""
+def handle_variables(params, boundary=None, dx=None, t_end=None):
+    """generic.py:159-286 for the keys the update reads, merged with the
+    configuration's test conditions (static/tests.py, ``configs``) as
+    astrea.py:116-119 merges them.  ``params`` is the flattened
+    parameters.yml mapping; invalid values revert to the defaults exactly as
+    the reference does (with a printed warning).  ``boundary``, ``dx`` and
+    ``t_end`` override the configuration's values when given."""
     out = {}
     for k, v in params.items():
         if k in ("checkpoints", "dimension"):
@@ -115,9 +120,12 @@ def handle_variables(params, boundary="edge", dx=None, t_end=0.2):
     perms = [a for a in itertools.permutations(range(d + 1)) if a[-1] == d]
     out["permutations"] = {i: a for i, a in enumerate(perms)}
     out["ortho_axis"] = perms[-1]
-    out["boundary"] = boundary
-    out["dx"] = dx if dx is not None else 1 / out["cells"]
-    out["t_end"] = t_end
+    tc = configs.test_conditions(out["config"], out["cells"], out["gamma"])
+    out["boundary"] = boundary if boundary is not None else tc["boundary"]
+    out["dx"] = dx if dx is not None else tc["dx"]
+    out["t_end"] = t_end if t_end is not None else tc["t_end"]
+    for k in ("start_pos", "end_pos", "shock_pos", "misc", "initial_left", "initial_right"):
+        out[k] = tc[k]
     return SimVariables(**{k: out[k] for k in SimVariables._fields})
 
 

@@ -5,6 +5,7 @@ Run once, in the development container only (the reference source tree is
 not available anywhere else):
 
     PYTHONDONTWRITEBYTECODE=1 python3 tests/golden/make_golden.py /root/reference
+    PYTHONDONTWRITEBYTECODE=1 python3 tests/golden/make_golden.py /root/reference --configs
 
 The reference has no tests and no reference-held vectors, so the functions
 that no closed form pins (the PPM 'mc' extrapolant limiter branch structure,
@@ -372,5 +373,48 @@ def main(ref):
     print("golden written")
 
 
+# configurations of static/tests.py written by write_configs: (config, dimension, gamma)
+CONFIG_CASES = [("sod", 1, 1.4), ("sedov", 1, 1.4), ("shu-osher", 1, 1.4), ("sin", 1, 1.4), ("gauss", 1, 1.4),
+                ("lin", 1, 5 / 3), ("lin mhd", 1, 5 / 3), ("slow", 1, 1.4), ("sq", 1, 1.4), ("rj", 1, 5 / 3),
+                ("bw", 1, 2.), ("toro1", 1, 1.4), ("toro2", 1, 1.4), ("toro3", 1, 1.4), ("toro4", 1, 1.4),
+                ("toro5", 1, 1.4), ("sedov", 2, 1.4), ("gauss", 2, 1.4), ("sod", 2, 1.4), ("khi", 2, 1.4),
+                ("ivc", 2, 1.4), ("ot", 2, 5 / 3), ("rotor", 2, 1.4), ("mhd blast", 2, 5 / 3)] + \
+               [(f"ll{k}", 2, 1.4) for k in range(1, 20)]
+
+
+def write_configs(ref):
+    """configs.npz: for every configuration of static/tests.py (CONFIG_CASES),
+    the table entries generate_test_conditions returns and the initial state
+    constructor.initialise builds from them (convert=True), for a point
+    (plm) and a 4th-order (ppm) scheme."""
+    sys.path.insert(0, ref)
+    from functions import fv, constructor
+    from static import tests as ref_tests
+    np.seterr(all='ignore')
+    out = {}
+    for config, dim, gamma in CONFIG_CASES:
+        N = 32 if dim == 1 else 12
+        tc = ref_tests.generate_test_conditions(config, N, gamma)
+        key = f"{config.replace(' ', '_')}_{dim}d"
+        for name in ("start_pos", "end_pos", "shock_pos", "t_end", "dx"):
+            out[f"{key}_{name}"] = np.float64(tc[name])
+        out[f"{key}_boundary"] = np.array(tc["boundary"])
+        out[f"{key}_left"], out[f"{key}_right"] = np.asarray(tc["initial_left"]), np.asarray(tc["initial_right"])
+        for subgrid in ("plm", "ppm"):
+            sv = make_sv(fv, subgrid, "llf", "ssprk(3,3)", dim, tc['boundary'], gamma, .5, N,
+                         magnetic_2d=(config in ("ot", "rotor", "mhd blast")), config=config)
+            sv = sv._replace(start_pos=tc['start_pos'], end_pos=tc['end_pos'], shock_pos=tc['shock_pos'],
+                             misc=tc['misc'], initial_left=tc['initial_left'],
+                             initial_right=tc['initial_right'], t_end=tc['t_end'], dx=tc['dx'])
+            out[f"{key}_{subgrid}_grid"] = constructor.initialise(sv, convert=True)
+    np.savez_compressed(os.path.join(HERE, "configs.npz"), **out)
+    print(f"configs.npz written: {len(CONFIG_CASES)} configurations")
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    ref = args[0] if args else "/root/reference"
+    if "--configs" in sys.argv:
+        write_configs(ref)
+    else:
+        main(ref)

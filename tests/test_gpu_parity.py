@@ -958,6 +958,50 @@ def test_save_run_layout(tmp_path):
                 assert np.array_equal(z[g], w)
 
 
+@pytest.mark.parametrize("params", [
+    {"config": "sod", "cells": 128, "subgrid": "plm", "solver": "hllc", "timestep": "ssprk(3,3)", "cfl": .8},
+    {"config": "toro2", "cells": 96, "subgrid": "ppm", "solver": "lf", "timestep": "ssprk(2,2)", "cfl": .5},
+    {"config": "ll3", "cells": 24, "dimension": 2, "subgrid": "plm", "solver": "hllc", "timestep": "ssprk(3,3)",
+     "cfl": .4, "max_steps": 8},
+    {"config": "khi", "cells": 24, "dimension": 2, "subgrid": "ppm", "solver": "hllc", "timestep": "ssprk(3,3)",
+     "cfl": .4, "max_steps": 8}])
+def test_simulate_configuration(params):
+    """core.simulate: the configuration table, the initial state and the
+    device loop together, against the oracle's loop on the same initial
+    state (same steps, rel L1 <= 1e-12)."""
+    from astrea_amd import core, ic
+    _say(f"[gpu] simulate {params['config']} {params['subgrid']} {params['solver']}")
+    out, st, sv = core.simulate(params)
+    s = O.Scheme(dimension=sv.dimension, subgrid=sv.subgrid, solver=sv.solver, timestep=sv.timestep,
+                 boundary=sv.boundary, gamma=sv.gamma, cfl=sv.cfl, dx=sv.dx)
+    r = O.run(ic.initialise(sv), s, sv.t_end, sv.checkpoints, params.get("max_steps"))
+    assert st["steps"] == r.steps and not st["failed"]
+    e = np.max(rel_l1(out, r.grid))
+    _say(f"[gpu]   {r.steps} steps, t={st['t']:.6g}, rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
+def test_simulate_fails_where_the_reference_raises():
+    """Lax-Liu 12 at 24^2, PPM/HLLC: in step 8 a Jacobian state turns
+    non-finite and the reference's LAPACK call raises (the oracle with
+    eig="lapack" raises there too); the device run stops failed after 7 steps
+    and keeps the state of step 7."""
+    from astrea_amd import core, ic
+    from dataclasses import replace
+    params = {"config": "ll12", "cells": 24, "dimension": 2, "subgrid": "ppm", "solver": "hllc",
+              "timestep": "ssprk(3,3)", "cfl": .4, "max_steps": 20}
+    _say("[gpu] simulate ll12 ppm hllc to failure")
+    out, st, sv = core.simulate(params)
+    assert st["failed"] and st["steps"] == 7
+    s = O.Scheme(dimension=2, subgrid="ppm", solver="hllc", timestep="ssprk(3,3)", boundary=sv.boundary,
+                 gamma=sv.gamma, cfl=sv.cfl, dx=sv.dx, eig="lapack")
+    q0 = ic.initialise(sv)
+    with pytest.raises(np.linalg.LinAlgError):
+        O.run(q0, s, sv.t_end, 1, 8)
+    r = O.run(q0, replace(s, eig="closed"), sv.t_end, 1, 7)
+    assert np.max(rel_l1(out, r.grid)) <= 1e-12
+
+
 def test_snapshot_capture_host_loop_and_graph_agree():
     """The same series three ways on a 2D problem: the host-stepped loop
     (run_saving), the step program's capture with eager chunks, and the
