@@ -601,8 +601,14 @@ struct RowAcc {
 // -> limited interface states wL/wR of the tile's own cells, written in the
 // sweep-frame layout [t][s] (s contiguous) of type R.
 // ============================================================================
+// Hydrodynamic tiles take 39 KB of LDS, room for four workgroups per CU; the
+// register budget is capped so that four waves fit each SIMD (the PPM
+// instantiations spill 80 bytes per lane to do so and still run 10 % faster
+// than at three waves on the 1024^2 shear layer).  MHD windows leave LDS for
+// three, so their registers stay uncapped.
 template <class R, int MASK, int RECON, int AX, int TS, int TT, bool SPLIT = false>
-__global__ __launch_bounds__(256) void recon2d_kernel(SweepArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VS<MASK>::NV * (int)sizeof(R) > 32 ? 1 : 4)))
+void recon2d_kernel(SweepArgs a) {
     using V = VS<MASK>;
     constexpr int NV = V::NV;
     constexpr bool HO = (RECON == PPM || RECON >= WENO3);
