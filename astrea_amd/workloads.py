@@ -58,14 +58,19 @@ def _start(plan, inits):
         plan.load(inits)
 
 
-def run_ensemble_batched(grids, probs, precision="fp64", chunk=16, use_graph=True, blocks=None, compat="physical"):
+def run_ensemble_batched(grids, probs, precision="fp64", chunk=32, use_graph=True, blocks=None, compat="physical",
+                         compact_every=4):
+    """The ensemble in one plan, replayed in graph chunks of `chunk` steps with
+    the tile lists compacted every `compact_every` chunks (on the 512-problem
+    ensemble, 32 and 4 run 7 % faster than 16 and 1: each compaction waits for
+    the device, while a finished problem's tiles only cost an early exit)."""
     plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
     if blocks:
         plan.set_blocks(blocks)
     _start(plan, grids)
     plan.reset()
     torch.cuda.synchronize()
-    st, secs, nch = batch.run(plan, chunk=chunk, use_graph=use_graph)
+    st, secs, nch = batch.run(plan, chunk=chunk, use_graph=use_graph, compact_every=compact_every)
     zu = sum(p["n0"] * s["steps"] for p, s in zip(probs, st))
     out = plan.state.clone()
     offs = plan.offsets

@@ -142,9 +142,9 @@ def bench_ensemble(args):
     blocks, tune = W.autotune_blocks(grids, probs, "fp64")
     res = {}
     for prec in ("fp64", "split"):
-        res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, chunk=16, blocks=blocks)
+        res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, blocks=blocks)
         _progress(f"[ensemble] batched {prec}: {res[prec]['zu_per_s']:.3e} ZU/s")
-    literal = W.run_ensemble_batched(grids, probs, precision="fp64", chunk=16, blocks=blocks, compat="astrea")
+    literal = W.run_ensemble_batched(grids, probs, precision="fp64", blocks=blocks, compat="astrea")
     ref = res["fp64"]
     errs, mism = [], 0
     for i in range(len(probs)):

@@ -616,20 +616,22 @@ def _ensemble(nprob, seed, nmin, nmax, t_scale=1 / 8):
     return grids, probs
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_equals_alone(use_graph):
+@pytest.mark.parametrize("use_graph,compact_every", [(False, 1), (True, 1), (True, 3)])
+def test_batch_equals_alone(use_graph, compact_every):
     """A ragged batch of seeded 1D Riemann problems (PPM, HLLC, SSPRK(3,3),
     CFL 0.5) in one plan gives every problem the same result, step count and
     failure status as running it alone (bitwise), with and without the
-    captured-graph replay and tile-list compaction."""
+    captured-graph replay, compacting the tile lists after every chunk or
+    every third."""
     from astrea_amd.plan import Plan
     from astrea_amd import batch
     grids, probs = _ensemble(12, 11, 32, 300)
-    _say(f"[gpu] batch of {len(grids)} problems, graph={use_graph}, sizes={[p['n0'] for p in probs]}")
+    _say(f"[gpu] batch of {len(grids)} problems, graph={use_graph}, compact_every={compact_every}, "
+         f"sizes={[p['n0'] for p in probs]}")
     plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13)
     plan.load(grids)
     plan.reset()
-    st, secs, nch = batch.run(plan, chunk=8, use_graph=use_graph)
+    st, secs, nch = batch.run(plan, chunk=8, use_graph=use_graph, compact_every=compact_every)
     outs = plan.unpack()
     plan.close()
     for i, (g, p) in enumerate(zip(grids, probs)):
