@@ -179,6 +179,18 @@ bool snapshot_any(const ah_plan* p, int prob, double* out, hipStream_t s, const 
     }
     return false;
 }
+// per-problem status written in stream order into device memory (ah_plan_status_async)
+__global__ void status_kernel(const ProbState* ps, int nprob, ah_problem_status* out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    ah_problem_status o;
+    o.t = ps[p].t;
+    o.dt = ps[p].dt;
+    o.steps = ps[p].steps;
+    o.active = ps[p].active;
+    o.failed = ps[p].failed;
+    out[p] = o;
+}
 }  // namespace
 
 extern "C" {
@@ -779,6 +791,14 @@ int ah_plan_time_import(ah_plan* p, const ah_time_state* in, void* stream) {
     }
     HIPCHK(hipMemcpyAsync(p->d_ps, h.data(), p->nprob * sizeof(ProbState), hipMemcpyHostToDevice, (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return AH_OK;
+}
+
+int ah_plan_status_async(ah_plan* p, ah_problem_status* out, void* stream) {
+    if (!p || !out) return AH_ERR_INVALID;
+    hipLaunchKernelGGL(status_kernel, dim3((p->nprob + 255) / 256), dim3(256), 0, (hipStream_t)stream, p->d_ps,
+                       p->nprob, out);
+    HIPCHK(hipGetLastError());
     return AH_OK;
 }
 

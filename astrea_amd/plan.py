@@ -260,6 +260,20 @@ class Plan:
         L.check(self.lib.ah_plan_status(self.h, st, self._stream()), "ah_plan_status")
         return [dict(t=s.t, dt=s.dt, steps=s.steps, active=bool(s.active), failed=bool(s.failed)) for s in st]
 
+    def status_async(self):
+        """Queue the statuses of this point of the stream (ah_plan_status_async
+        into device memory, then a copy to pinned host memory) and return at
+        once; ``PendingStatus.result()`` waits for that point only."""
+        n = self.nprob * ctypes.sizeof(L.AhProblemStatus)
+        dev = torch.empty(n, dtype=torch.uint8, device=self.device)
+        L.check(self.lib.ah_plan_status_async(self.h, ctypes.c_void_p(dev.data_ptr()), self._stream()),
+                "ah_plan_status_async")
+        host = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        host.copy_(dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        return PendingStatus(host, dev, ev, self.nprob)
+
     def run(self, chunk=32, max_chunks=10 ** 9):
         """Step until no problem is active; the host looks at the device time
         state once per chunk of steps."""
@@ -312,3 +326,18 @@ class Plan:
                                       ctypes.c_void_p(f1.data_ptr()), ctypes.c_void_p(L_.data_ptr()),
                                       self._stream()), "ah_compute_L")
         return L_
+
+
+class PendingStatus:
+    """Statuses queued by ``Plan.status_async``."""
+
+    def __init__(self, host, dev, event, nprob):
+        self._host, self._dev, self._event, self._n = host, dev, event, nprob
+
+    def ready(self):
+        return self._event.query()
+
+    def result(self):
+        self._event.synchronize()
+        arr = (L.AhProblemStatus * self._n).from_buffer_copy(self._host.numpy().tobytes())
+        return [dict(t=s.t, dt=s.dt, steps=s.steps, active=bool(s.active), failed=bool(s.failed)) for s in arr]

@@ -230,6 +230,13 @@ int ah_plan_time_import(ah_plan* plan, const ah_time_state* in, void* stream);
 /* Copy the per-problem time state to the host (synchronises the stream). */
 int ah_plan_status(ah_plan* plan, ah_problem_status* out, void* stream);
 
+/* The same statuses written by a kernel, in stream order, into `out` (device
+ * memory, nprob entries) without synchronising: a caller keeps stepping and
+ * reads the failure flags and times of that point of the stream when its copy
+ * arrives (the reference has no asynchronous path; its loop checks nothing
+ * until an exception). */
+int ah_plan_status_async(ah_plan* plan, ah_problem_status* out, void* stream);
+
 /* Parity entry points mirroring the reference's operator interface, for one
  * problem (index `prob`) of a bound plan, at step parity `parity` (0: axis-0
  * sweep first, as on even steps; 1: reversed, astrea.py:85).

@@ -1084,6 +1084,34 @@ def test_simulate_fails_where_the_reference_raises():
     assert np.max(rel_l1(out, r.grid)) <= 1e-12
 
 
+def test_status_async_matches_status():
+    """ah_plan_status_async: statuses queued in stream order between chunks
+    of steps, read after the whole run was queued, equal the synchronous ones
+    of a second plan read at the same points."""
+    from astrea_amd.plan import Plan
+    grids, probs = _ensemble(6, 11, 32, 200)
+    _say(f"[gpu] status_async, {len(probs)} problems")
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13)
+    plan.load(grids)
+    plan.reset()
+    pend = []
+    for _ in range(6):
+        plan.step(20)
+        pend.append(plan.status_async())
+    plan.step(20)
+    final = plan.status()
+    assert pend[-1].ready()
+    assert pend[0].result() != final or all(not s["active"] for s in final)
+    again = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13)
+    again.load(grids)
+    again.reset()
+    for k in range(6):
+        again.step(20)
+        assert again.status() == pend[k].result(), k
+    plan.close()
+    again.close()
+
+
 def test_snapshot_capture_host_loop_and_graph_agree():
     """The same series three ways on a 2D problem: the host-stepped loop
     (run_saving), the step program's capture with eager chunks, and the
