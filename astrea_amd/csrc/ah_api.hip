@@ -465,7 +465,7 @@ static int blocks_for(const ah_plan* p, const TileList& L) { return std::min<int
 
 // L of the stage input X for every active problem (evolve_space + compute_L)
 static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, int parity_override,
-                         double* flux0, double* flux1, hipStream_t s) {
+                         double* flux0, double* flux1, hipStream_t s, bool fuse33 = false) {
     SweepArgs a = base_args(p);
     a.X = X;
     a.stage0 = stage0;
@@ -495,6 +495,11 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.ntiles_dev = p->tl[3 + ax].d_count;
         a.last = ax == 1;
         a.out = ax == 0 ? p->Dbuf : Lout;
+        if (ax == 1 && fuse33) {
+            a.fuse_k2 = p->regs[2];
+            a.fuse_u = p->regs[0];
+            a.fuse_k1 = p->regs[1];
+        }
         a.Dprev = p->Dbuf;
         a.flux_out = ax == 0 ? flux0 : flux1;
         a.flux_plane = ax == 0 ? (long long)(p->probs[0].n0 + 1) * p->probs[0].n1
@@ -561,7 +566,10 @@ static void launch_step(ah_plan* p, hipStream_t s) {
     const bool mhd = p->sch.magnetic_2d != 0;
     for (size_t st = 0; st < p->prog.size(); ++st) {
         const Stage& S = p->prog[st];
-        launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s);
+        // the middle SSPRK(3,3) stage combines inside the second sweep's flux
+        // kernel (dt is known after stage 0; a failed problem's K2 is never read)
+        const bool fuse33 = S.op == OP_33_1 && !mhd && p->sch.dims == 2;
+        launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s, fuse33);
         MhdArgs m;
         if (mhd) {
             // corner EMF from both sweeps' face values; it replaces the B rows of L
@@ -577,7 +585,7 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
         if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[7]), s);
-        launch_combine(p, S.op, s);
+        if (!fuse33) launch_combine(p, S.op, s);
         if (mhd) {
             double* K = p->regs[target_reg(S.op)];
             m.K = K;

@@ -58,6 +58,11 @@ struct SweepArgs {
     int parity_override;  // -1: use per-problem parity
     int last;             // 2D: this flux launch finishes L
     int compat;           // 0: reference solver axis (enumeration index), 1: physical axis
+    // 2D second sweep of the middle SSPRK(3,3) stage: write K2 = (3U + K1 + dt L) / 4
+    // (evolvers.py:172) in place of L, in the combine kernel's operation order
+    double* fuse_k2;      // null: write L
+    const double* fuse_u;
+    const double* fuse_k1;
 };
 
 __device__ inline int clampi(int p, int lo, int hi) { return p < lo ? lo : (p > hi ? hi : p); }
@@ -1066,10 +1071,18 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const double d = double(bydx(R(F_s[v][itt][is + 1] - F_s[v][itt][is])));
-            if (a.last)
-                a.out[v * P + g] = -(a.Dprev[v * P + g] + d);
-            else
+            if (a.last) {
+                const double L = -(a.Dprev[v * P + g] + d);
+                if (a.fuse_k2) {
+#pragma clang fp contract(off)
+                    const long long j = v * P + g;
+                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + ld_const(&ps->dt) * L);
+                } else {
+                    a.out[v * P + g] = L;
+                }
+            } else {
                 a.out[v * P + g] = d;
+            }
         }
     }
     __syncthreads();  // the next tile reuses the LDS windows
