@@ -470,6 +470,11 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
     a.X = X;
     a.stage0 = stage0;
     a.parity_override = parity_override;
+    if (fuse33) {
+        a.fuse_k2 = p->regs[2];
+        a.fuse_u = p->regs[0];
+        a.fuse_k1 = p->regs[1];
+    }
     if (p->sch.dims == 1) {
         a.out = Lout;
         a.tiles = p->tl[0].d;
@@ -495,11 +500,8 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.ntiles_dev = p->tl[3 + ax].d_count;
         a.last = ax == 1;
         a.out = ax == 0 ? p->Dbuf : Lout;
-        if (ax == 1 && fuse33) {
-            a.fuse_k2 = p->regs[2];
-            a.fuse_u = p->regs[0];
-            a.fuse_k1 = p->regs[1];
-        }
+        if (ax == 0) a.fuse_k2 = nullptr;  // the first sweep writes its D
+        else if (fuse33) a.fuse_k2 = p->regs[2];
         a.Dprev = p->Dbuf;
         a.flux_out = ax == 0 ? flux0 : flux1;
         a.flux_plane = ax == 0 ? (long long)(p->probs[0].n0 + 1) * p->probs[0].n1
@@ -566,9 +568,10 @@ static void launch_step(ah_plan* p, hipStream_t s) {
     const bool mhd = p->sch.magnetic_2d != 0;
     for (size_t st = 0; st < p->prog.size(); ++st) {
         const Stage& S = p->prog[st];
-        // the middle SSPRK(3,3) stage combines inside the second sweep's flux
-        // kernel (dt is known after stage 0; a failed problem's K2 is never read)
-        const bool fuse33 = S.op == OP_33_1 && !mhd && p->sch.dims == 2;
+        // the middle SSPRK(3,3) stage combines inside the kernel that finishes L
+        // (the 1D sweep, the second 2D flux sweep): dt is known after stage 0,
+        // and a failed problem's K2 is never read
+        const bool fuse33 = S.op == OP_33_1 && !mhd;
         launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s, fuse33);
         MhdArgs m;
         if (mhd) {

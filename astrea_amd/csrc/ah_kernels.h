@@ -571,7 +571,14 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const R d = bydx(R(fl[v][i + 1] - fl[v][i]));
-            a.out[v * P + off + cell] = -double(d);
+            const double L = -double(d);
+            const long long j = v * P + off + cell;
+            if (a.fuse_k2) {
+#pragma clang fp contract(off)
+                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + ld_const(&ps->dt) * L);
+            } else {
+                a.out[j] = L;
+            }
         }
     }
     __syncthreads();  // the next tile reuses the LDS windows
