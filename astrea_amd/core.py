@@ -63,7 +63,7 @@ def run_captured(plan, save, t_end, every="step", chunk=16, i=0):
     if st["t"] <= t_end and not st["failed"]:
         save(str(float(st["t"])), plan.snapshot(i).cpu().numpy())
     plan.capture(i, every, slots=chunk)
-    host = torch.empty(tuple(plan._cap["ring"].shape), dtype=torch.float64, pin_memory=True)
+    host = torch.empty((chunk,) + plan.shape(i)[::-1] + (8,), dtype=torch.float64, pin_memory=True)
     stream = torch.cuda.current_stream(plan.device)
     try:
         plan.step(chunk)

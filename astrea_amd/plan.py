@@ -210,8 +210,10 @@ class Plan:
         after every step (every="step") or after the steps whose dt was
         clamped to a checkpoint (every="checkpoint"), into a device ring of
         `slots` snapshots; every=None stops it.  ``drain`` reads them."""
-        mode = {None: L.AH_CAPTURE_OFF, "step": L.AH_CAPTURE_EVERY_STEP,
-                "checkpoint": L.AH_CAPTURE_CHECKPOINTS}[every]
+        modes = {None: L.AH_CAPTURE_OFF, "step": L.AH_CAPTURE_EVERY_STEP, "checkpoint": L.AH_CAPTURE_CHECKPOINTS}
+        if every not in modes:
+            raise ValueError(f"every must be one of {list(modes)}")
+        mode = modes[every]
         if mode == L.AH_CAPTURE_OFF:
             L.check(self.lib.ah_plan_set_capture(self.h, int(i), mode, None, 0, None, None, self._stream()),
                     "ah_plan_set_capture")
