@@ -513,6 +513,8 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
 
 static void launch_combine(ah_plan* p, int op, hipStream_t s) {
     CombineArgs c;
+    c.live_dt = p->sch.magnetic_2d ? 0 : 1;
+    c.dims = p->sch.dims;
     for (int r = 0; r < 9; ++r) c.regs[r] = p->regs[r];
     c.L = p->Lbuf;
     c.pd = p->d_pd;
@@ -583,7 +585,9 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             p->ks.corner(m, blocks_for(p, p->tl[7]), s);
             p->ks.magL(m, nc, s);
         }
-        if (st == 0) {
+        // with constrained transport dt is stored after stage 0; otherwise every
+        // kernel of the step evaluates it (step_dt) and finish_kernel stores it
+        if (st == 0 && mhd) {
             ProfScope ps_(p, AH_K_DT, s);
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
@@ -598,7 +602,10 @@ static void launch_step(ah_plan* p, hipStream_t s) {
     }
     {
         ProfScope ps_(p, AH_K_ADVANCE, s);
-        hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+        if (mhd)
+            hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+        else
+            hipLaunchKernelGGL(finish_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
     }
     if (p->cap.ctl) {
         snapshot_any(p, p->cap.prob, nullptr, s, p->cap);

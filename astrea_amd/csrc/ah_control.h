@@ -56,6 +56,43 @@ __global__ void advance_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
     s.active = more_time && more_steps;
 }
 
+// End of a step whose kernels evaluated dt themselves (step_dt): the state
+// updates of dt_kernel and advance_kernel in one launch, with the same
+// outcome -- a failure seen at stage 0 (non-finite radii) leaves dt and the
+// checkpoint index as they were, a later one records this step's dt.
+__global__ void finish_kernel(const ProbDesc* pd, ProbState* ps, int nprob, int dims) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    ProbState& s = ps[p];
+    if (!s.active) return;
+    const ProbDesc d = pd[p];
+    const double e0 = __longlong_as_double((long long)s.eig[0]);
+    const double e1 = dims == 2 ? __longlong_as_double((long long)s.eig[1]) : e0;
+    if (!(e0 <= 1.7976931348623157e308) || !(e1 <= 1.7976931348623157e308)) {
+        s.eig[0] = s.eig[1] = 0ull;
+        s.failed = 1;
+        s.active = 0;
+        return;
+    }
+    int hit;
+    const double dt = step_dt(d, s, dims, &hit);
+    s.eig[0] = s.eig[1] = 0ull;
+    s.idx += hit;
+    s.ckpt = hit;
+    s.dt = dt;
+    if (s.fail_flag) {
+        s.failed = 1;
+        s.active = 0;
+        return;
+    }
+    s.t = s.t + s.dt;
+    s.steps += 1;
+    s.parity ^= 1;
+    const bool more_time = (s.t <= d.t_end) && !(s.t == d.t_end);
+    const bool more_steps = d.max_steps < 0 || s.steps < d.max_steps;
+    s.active = more_time && more_steps;
+}
+
 __global__ void reset_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nprob) return;
@@ -100,6 +137,8 @@ struct CombineArgs {
     long long plane;
     int nv;
     int op;
+    int live_dt;   // 1: dt from step_dt (no dt launch in this step program), 0: the stored dt
+    int dims;
 };
 
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
@@ -109,7 +148,13 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
         const Tile tl = ld_const(a.tiles + ti);
         const ProbState& s = a.ps[tl.prob];
         if (!ld_const(&s.active) || ld_const(&s.fail_flag)) continue;
-        const double dt = ld_const(&s.dt);
+        double dt;
+        if (a.live_dt) {
+            int hit;
+            dt = step_dt(ld_const(&a.pd[tl.prob]), s, a.dims, &hit);
+        } else {
+            dt = ld_const(&s.dt);
+        }
         const long long base = ld_const(&a.pd[tl.prob].off) + tl.s0;
         for (int c = threadIdx.x; c < tl.t0; c += blockDim.x)
         for (int v = 0; v < a.nv; ++v) {

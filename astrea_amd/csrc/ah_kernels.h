@@ -36,6 +36,23 @@ struct Tile {
     int prob, s0, t0, pad;
 };
 
+// This step's dt from the stage-0 spectral radii (astrea.py:70-78): the CFL
+// value, clamped to the next checkpoint (*hit = 1 then).  A pure function of
+// the step's time state, so every kernel of the step evaluates the same dt.
+__device__ inline double step_dt(const ProbDesc& d, const ProbState& s, int dims, int* hit) {
+#pragma clang fp contract(off)
+    const double e0 = __longlong_as_double((long long)s.eig[0]);
+    const double e1 = dims == 2 ? __longlong_as_double((long long)s.eig[1]) : e0;
+    const double a0 = d.dx / e0, a1 = d.dx / e1;
+    double dt = d.cfl * (dims == 2 ? (a1 < a0 ? a1 : a0) : a0);
+    *hit = 0;
+    if (s.t + dt >= d.chkpt * s.idx) {
+        dt = d.chkpt * s.idx - s.t;
+        *hit = 1;
+    }
+    return dt;
+}
+
 struct SweepArgs {
     const double* X;      // stage input state, packed SoA (plane stride `plane`)
     double* out;          // 1D: L;  2D flux: D (first sweep) or L (second)
@@ -575,7 +592,9 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             const long long j = v * P + off + cell;
             if (a.fuse_k2) {
 #pragma clang fp contract(off)
-                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + ld_const(&ps->dt) * L);
+                int hit;
+                const double dt = step_dt(pd, *ps, 1, &hit);
+                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + dt * L);
             } else {
                 a.out[j] = L;
             }
@@ -1083,7 +1102,9 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
                 if (a.fuse_k2) {
 #pragma clang fp contract(off)
                     const long long j = v * P + g;
-                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + ld_const(&ps->dt) * L);
+                    int hit;
+                    const double dt = step_dt(pd, *ps, 2, &hit);
+                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + dt * L);
                 } else {
                     a.out[v * P + g] = L;
                 }
