@@ -582,6 +582,8 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     __syncthreads();
     // F: L = -(F[i+1] - F[i]) / dx (evolvers.py:41-60)
     const DivDx bydx(pd.dx);
+    int hit_;
+    const double fdt = a.fuse_k2 ? step_dt(pd, *ps, 1, &hit_) : 0.0;
     for (int i = threadIdx.x; i < TS; i += blockDim.x) {
         const int cell = s0 + i;
         if (cell >= N) break;
@@ -592,9 +594,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             const long long j = v * P + off + cell;
             if (a.fuse_k2) {
 #pragma clang fp contract(off)
-                int hit;
-                const double dt = step_dt(pd, *ps, 1, &hit);
-                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + dt * L);
+                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + fdt * L);
             } else {
                 a.out[j] = L;
             }
@@ -1087,6 +1087,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     }
     // E: flux difference; first sweep writes D, second writes L = -(D0 + D1)
     const DivDx bydx(pd.dx);
+    int hit_;
+    const double fdt = a.fuse_k2 ? step_dt(pd, *ps, 2, &hit_) : 0.0;
     for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
         int itt, is;
         if constexpr (AX == 0) { is = i / TT; itt = i % TT; }
@@ -1102,9 +1104,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
                 if (a.fuse_k2) {
 #pragma clang fp contract(off)
                     const long long j = v * P + g;
-                    int hit;
-                    const double dt = step_dt(pd, *ps, 2, &hit);
-                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + dt * L);
+                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + fdt * L);
                 } else {
                     a.out[v * P + g] = L;
                 }
