@@ -259,7 +259,10 @@ int ah_compute_L(ah_plan* plan, int prob, int parity, const double* flux0, const
  * astrea.py:178-180).  ah_plan_profile(plan, capacity) enables recording of up
  * to `capacity` launches (0 disables); ah_plan_profile_read synchronises,
  * returns the summed milliseconds and launch counts per kernel kind and
- * clears the record. */
+ * clears the record.  Without constrained transport a step has no dt launch
+ * (its kernels evaluate dt) and AH_K_ADVANCE times the end-of-step launch
+ * that stores dt and advances t; the middle SSPRK(3,3) stage has no
+ * combination launch (the launch that finishes its L writes K2). */
 enum ah_kernel_kind {
     AH_K_SWEEP1D = 0, AH_K_RECON0 = 1, AH_K_FLUX0 = 2, AH_K_RECON1 = 3, AH_K_FLUX1 = 4,
     AH_K_COMBINE = 5, AH_K_DT = 6, AH_K_ADVANCE = 7, AH_NKIND = 8
