@@ -378,7 +378,8 @@ CONFIG_CASES = [("sod", 1, 1.4), ("sedov", 1, 1.4), ("shu-osher", 1, 1.4), ("sin
                 ("lin", 1, 5 / 3), ("lin mhd", 1, 5 / 3), ("slow", 1, 1.4), ("sq", 1, 1.4), ("rj", 1, 5 / 3),
                 ("bw", 1, 2.), ("toro1", 1, 1.4), ("toro2", 1, 1.4), ("toro3", 1, 1.4), ("toro4", 1, 1.4),
                 ("toro5", 1, 1.4), ("sedov", 2, 1.4), ("gauss", 2, 1.4), ("sod", 2, 1.4), ("khi", 2, 1.4),
-                ("ivc", 2, 1.4), ("ot", 2, 5 / 3), ("rotor", 2, 1.4), ("mhd blast", 2, 5 / 3)] + \
+                ("ivc", 2, 1.4), ("ot", 2, 5 / 3), ("rotor", 2, 1.4), ("mhd blast", 2, 5 / 3),
+                ("isentropic-vortex", 2, 1.4), ("lax-liu 5", 2, 1.4), ("sod shock", 1, 1.4), ("sedov blast", 1, 1.4)] + \
                [(f"ll{k}", 2, 1.4) for k in range(1, 20)]
 
 
