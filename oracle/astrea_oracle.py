@@ -50,7 +50,13 @@ PARITY = {
     "MHD constrained transport": "golden space_2d.npz + steps ot16_*",
     "SSPRK/RK stages": "golden steps rk_* + amplification polynomials",
     "driver loop / dt": "golden steps dts",
-    "ppm flattener": "parity unpinned (reference crashes, schemes/ppm.py:67); invariant eta=1 only",
+    "ppm flattener (astrea)": "golden pins.npz flat_* (reference apply_flattener called in isolation, "
+                              "1D/2D, both axes and boundaries, exact switch points) + eta=1 invariant",
+    "ppm flattener (colella90)": "Colella's shock test: smooth data unflattened, eta<1 only where it fires",
+    "hllc compat=physical": "golden pins.npz mirror_* (y sweep = reference x sweep of the mirrored grid), "
+                            "space_2d x sweep, consistency at axis 1, upwind supersonic limits, "
+                            "transposition equivariance, convergence to the exact Riemann solution",
+    "hlld compat=physical": "golden space_2d p0 (the reference's call at the physical axis)",
     "artificial viscosity": "parity unpinned (reference output never read, schemes/ppm.py:95)",
 }
 
@@ -382,13 +388,15 @@ def mc_extrapolant_limiter(wS, bc, wF_L, wF_R):
 def flattener_eta(wS, axis, bc, variant, delta=.33, z0=.75, z1=.85):
     """PPM slope flattener (schemes/ppm.py:111-134, Colella 1990).
 
-    The reference's blend at ppm.py:67 cannot run (broadcast error for N != 8)
-    and no option reaches it, so this is a restatement of the documented
-    intent with eta per cell applied to all variables.  ``variant='astrea'``
-    keeps the literal condition of ppm.py:124 (eta~ := 0 where the flow is
-    expanding AND the pressure jump is weak); ``'colella90'`` zeroes eta~
-    unless the zone is compressive AND the jump strong, i.e. keeps the
-    reference value only where Colella's shock test fires."""
+    ``variant='astrea'`` is the reference's apply_flattener (its column 0:
+    the reference broadcasts eta over the 8 variables), bitwise equal to it
+    on the pins.npz vectors.  The reference's blend at ppm.py:67 cannot run
+    (broadcast error for N != 8) and no option reaches it, so the blend in
+    ``reconstruct`` applies eta per cell to all variables.  The literal
+    condition of ppm.py:124 sets eta~ := 0 where the flow is expanding AND
+    the pressure jump is weak; ``'colella90'`` sets eta~ := 1 unless the zone
+    is compressive AND the jump strong, i.e. keeps the ratio ramp only where
+    Colella's shock test fires."""
     p1, p2 = pad(wS[..., 4], bc, 1), pad(wS[..., 4], bc, 2)
     u1 = pad(wS[..., axis + 1], bc, 1)
     dp1 = p1[2:] - p1[:-2]

@@ -6,6 +6,7 @@ not available anywhere else):
 
     PYTHONDONTWRITEBYTECODE=1 python3 tests/golden/make_golden.py /root/reference
     PYTHONDONTWRITEBYTECODE=1 python3 tests/golden/make_golden.py /root/reference --configs
+    PYTHONDONTWRITEBYTECODE=1 python3 tests/golden/make_golden.py /root/reference --pins
 
 The reference has no tests and no reference-held vectors, so the functions
 that no closed form pins (the PPM 'mc' extrapolant limiter branch structure,
@@ -412,10 +413,148 @@ def write_configs(ref):
     print(f"configs.npz written: {len(CONFIG_CASES)} configurations")
 
 
+def flattener_pressure_cases():
+    """1D primitive states (N = 24) whose pressures hit the flattener's exact
+    switch points: zeta = |dp1|/|dp2| = 3/4 (z0) and 17/20 (rounds to the
+    literal .85, z1), flat pressure (dp2 = 0, the guarded 0/0), dp1 = 0 with
+    dp2 != 0, and both signs of the velocity difference."""
+    cases = []
+    p = np.array([1., 1., 1., 1., 2., 3.5, 5., 5., 5., 5.,        # flat, then a 3/4 ramp
+                  5., 6., 9., 23., 26., 26., 26., 20., 14., 3.,    # 17/20 ramp and a falling jump
+                  3., 3., 4., 3.])
+    for useq in (0, 1, 2):
+        w = np.zeros((p.size, 8))
+        w[:, 0] = 1. + .1 * np.arange(p.size) % 3
+        w[:, 4] = p
+        if useq == 0:
+            w[:, 1] = -np.linspace(0., 2., p.size)      # compressive everywhere
+        elif useq == 1:
+            w[:, 1] = np.linspace(0., 2., p.size)       # expanding everywhere
+        else:
+            w[:, 1] = np.where(np.arange(p.size) % 4 < 2, .5, -.5)  # mixed signs
+        cases.append(w)
+    return cases
+
+
+def write_pins(ref):
+    """pins.npz: outputs of reference functions that no trajectory reaches,
+    computed in isolation on seeded inputs:
+
+    * schemes.ppm.apply_flattener (ppm.py:111-134) -- it runs on its own;
+      only the blend that would use it (ppm.py:67) cannot run;
+    * the x sweep of num_methods.evolvers.evolve_space on mirrored 2D grids
+      (axes swapped, v_x <-> v_y), whose HLLC solve at axis 0 is what a
+      physical-axis HLLC computes in the y sweep of the unmirrored grid;
+    * num_methods.limiters.interface_limiter (limiters.py:53-79) and
+      schemes.ppm.run with author 'c' and 'ph' (ppm.py:43-62 with the
+      Colella/Peterson-Hammett branches of limiters.py:144-201);
+    * schemes.ppm.apply_artificial_viscosity (ppm.py:138-170), which runs
+      only when the cell count equals the 8 variables (its broadcast)."""
+    sys.path.insert(0, ref)
+    from functions import fv
+    from num_methods import evolvers, limiters
+    from schemes import ppm
+    np.seterr(all='ignore')
+    rng = np.random.default_rng(20261203)
+    out = {}
+
+    # ---- flattener ------------------------------------------------------------
+    def flat(name, wS, axis, bc):
+        chi = ppm.apply_flattener(np.copy(wS), axis, bc)
+        assert np.all(chi == chi[..., :1]), "flattener output differs between variables"
+        out[f"flat_{name}_wS"] = wS
+        out[f"flat_{name}_eta"] = np.ascontiguousarray(chi[..., 0])
+        out[f"flat_{name}_meta"] = np.array([axis, 1 if bc == "wrap" else 0])
+
+    for bc in ("edge", "wrap"):
+        for k in range(3):
+            flat(f"rand{k}_{bc}", rand_1d_prim(rng, 32), 0, bc)
+        for k, w in enumerate(flattener_pressure_cases()):
+            flat(f"exact{k}_{bc}", w, 0, bc)
+        # 2D sweep-frame states: the shear layer and a quadrant problem, both sweeps
+        for tag, w in (("khi", khi_prim(rng, 16)), ("quad", quadrant_prim(rng, 16))):
+            w = np.copy(w)
+            w[..., 4] *= 1 + .4 * rng.random(w.shape[:2])   # pressure structure for the ratios
+            for axis in (0, 1):
+                wS = w if axis == 0 else np.ascontiguousarray(w.transpose(1, 0, 2))
+                flat(f"{tag}_ax{axis}_{bc}", wS, axis, bc)
+
+    # ---- HLLC on mirrored grids (physical-axis pin) ---------------------------
+    for subgrid, bc, maker in (("ppm", "wrap", khi_prim), ("plm", "edge", quadrant_prim), ("ppm", "edge", quadrant_prim)):
+        N = 12
+        sv = make_sv(fv, subgrid, "hllc", "ssprk(3,3)", 2, bc, 1.4, .4, N, t_end=10.)
+        w = maker(rng, N)
+        if maker is quadrant_prim:
+            w[..., 1:3] *= .3      # subsonic: every interface has s_L <= 0 <= s_R
+        grid = prim_to_avg_cons(fv, sv, w)
+        mir = np.ascontiguousarray(grid.transpose(1, 0, 2)[..., [0, 2, 1, 3, 4, 5, 6, 7]])
+        fl = evolvers.evolve_space(np.copy(mir), sv)
+        key = f"mirror_{subgrid}_{bc}"
+        out[key + "_grid"] = grid
+        out[key + "_mirror_flux_ax0"] = fl[(0, 1, 2)]['flux']
+        out[key + "_mirror_eigmax_ax0"] = np.array(fl[(0, 1, 2)]['eigmax'])
+
+    # ---- interface limiter and the other PPM authors --------------------------
+    for bc in ("edge", "wrap"):
+        for k in range(2):
+            wS = rand_1d_prim(rng, 32, mhd=(k == 1))
+            w2 = fv.add_boundary(wS, bc, 2)
+            w = np.copy(w2[1:-1])
+            wF = 7 / 12 * (wS + w[2:]) - 1 / 12 * (w[:-2] + w2[4:])
+            out[f"iflim_rand{k}_{bc}_wS"] = wS
+            out[f"iflim_rand{k}_{bc}_out"] = limiters.interface_limiter(np.copy(wF), w[:-2], wS, w[2:], w2[4:])
+        # monotone data: no local extremum anywhere, the face values pass through
+        wS = np.zeros((32, 8))
+        wS[:, 0] = 1 + np.arange(32) ** 2 / 100.
+        wS[:, 4] = 2 + np.arange(32) / 7.
+        w2 = fv.add_boundary(wS, bc, 2)
+        w = np.copy(w2[1:-1])
+        wF = 7 / 12 * (wS + w[2:]) - 1 / 12 * (w[:-2] + w2[4:])
+        out[f"iflim_mono_{bc}_wS"] = wS
+        out[f"iflim_mono_{bc}_out"] = limiters.interface_limiter(np.copy(wF), w[:-2], wS, w[2:], w2[4:])
+        for author in ("c", "ph"):
+            for mhd in (False, True):
+                N = 32
+                sv = make_sv(fv, "ppm", "llf", "ssprk(3,3)", 1, bc, 5 / 3 if mhd else 1.4, .5, N)
+                grid = sv.convert_primitive(rand_1d_prim(rng, N, mhd=mhd), sv)
+                try:
+                    data = ppm.run(np.copy(grid), sv, author=author)
+                except Exception as e:    # record, do not hide
+                    out[f"ppm_{author}_{'mhd' if mhd else 'hd'}_{bc}_error"] = np.array(repr(e))
+                    continue
+                (axes, d), = data.items()
+                key = f"ppm_{author}_{'mhd' if mhd else 'hd'}_{bc}"
+                out[key + "_grid"] = grid
+                out[key + "_wplus"], out[key + "_wminus"] = d['wFs']
+
+    # ---- artificial viscosity (runs only for 8 cells) --------------------------
+    for bc in ("edge", "wrap"):
+        for k, name in enumerate(("strong", "rand")):
+            N = 8
+            sv = make_sv(fv, "ppm", "llf", "ssprk(3,3)", 1, bc, 1.4, .5, N)
+            w = np.zeros((N, 8))
+            if name == "strong":
+                # compression strong enough that min(1, (dx lambda)^2/(beta c_min^2)) = 1 on every face
+                w[:, 0] = 1.
+                w[:, 1] = -5. * np.arange(N)
+                w[:, 4] = .1
+            else:
+                w[:] = rand_1d_prim(rng, N)
+            out[f"av_{name}_{bc}_wS"] = w
+            out[f"av_{name}_{bc}_mu"] = ppm.apply_artificial_viscosity(np.copy(w), 0, sv)
+    np.savez_compressed(os.path.join(HERE, "pins.npz"), **out)
+    with open(os.path.join(HERE, "pins_meta.json"), "w") as f:
+        json.dump(dict(numpy=np.__version__, reference="mervyzr/astrea 1.3.2 (setup.py:22-24)", seed=20261203,
+                       arrays=len(out)), f, indent=1)
+    print(f"pins.npz written: {len(out)} arrays")
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     ref = args[0] if args else "/root/reference"
     if "--configs" in sys.argv:
         write_configs(ref)
+    elif "--pins" in sys.argv:
+        write_pins(ref)
     else:
         main(ref)

@@ -1,0 +1,276 @@
+"""Pins for the oracle functions that no reference trajectory reaches.
+
+* The PPM flattener (schemes/ppm.py:111-134): the reference's
+  apply_flattener runs in isolation (only the blend at ppm.py:67 cannot run),
+  so pins.npz holds its outputs on 1D and 2D sweep-frame states, both
+  boundaries and both sweep axes, including the exact switch points of its
+  pressure ratio.  The colella90 variant is checked against what Colella's
+  shock test fixes.
+* The physical-axis HLLC (the reference's solvers.py:92-138 given the sweep's
+  own axis, with the star momentum at that axis and the upwind flux where
+  every wave moves right): its y sweep equals the reference's own x sweep of
+  the mirrored grid (pins.npz), its x sweep equals the reference's (space_2d),
+  it is consistent at axis 1 and upwind in the supersonic limits, 2D runs are
+  transposition-equivariant, and it converges to the exact Riemann solution.
+* The physical-axis HLLD: the reference's function at the physical axis,
+  which at odd step parity is the parity-0 call the reference makes
+  (space_2d fixtures).
+"""
+import numpy as np
+import pytest
+
+from oracle import astrea_oracle as O
+from tests import exact_riemann as X
+from tests._golden import load
+
+PINS = load("pins.npz")
+FLAT = sorted(k[:-3] for k in PINS.files if k.startswith("flat_") and k.endswith("_wS"))
+
+
+# ---- flattener --------------------------------------------------------------
+@pytest.mark.parametrize("name", FLAT)
+def test_flattener_equals_reference(name):
+    axis, wrap = PINS[name + "_meta"]
+    eta = O.flattener_eta(PINS[name + "_wS"], int(axis), "wrap" if wrap else "edge", "astrea")
+    assert np.array_equal(eta, PINS[name + "_eta"])
+
+
+def test_flattener_fixture_hits_the_switch_points():
+    """The constructed cases contain zeta exactly at z0 = .75 and at the
+    literal .85, flat pressure (0/0 ratio) and pressure jumps without a
+    centred difference, so each branch of the ratio is pinned."""
+    seen = set()
+    for name in FLAT:
+        if "exact" not in name:
+            continue
+        wS = PINS[name + "_wS"]
+        bc = "wrap" if PINS[name + "_meta"][1] else "edge"
+        p1, p2 = O.pad(wS[:, 4], bc, 1), O.pad(wS[:, 4], bc, 2)
+        dp1, dp2 = p1[2:] - p1[:-2], p2[4:] - p2[:-4]
+        z = O.gdiv(np.abs(dp1), np.abs(dp2))
+        seen |= {"z0"} if np.any(z == .75) else set()
+        seen |= {"z1"} if np.any(z == .85) else set()
+        seen |= {"flat"} if np.any((dp1 == 0) & (dp2 == 0)) else set()
+        seen |= {"dp1=0"} if np.any((dp1 == 0) & (dp2 != 0)) else set()
+        eta = PINS[name + "_eta"]
+        seen |= {"partial"} if np.any((eta > 0) & (eta < 1)) else set()
+    assert seen == {"z0", "z1", "flat", "dp1=0", "partial"}
+
+
+@pytest.mark.parametrize("bc", ["edge", "wrap"])
+def test_flattener_eta_one_is_plain_ppm(bc, monkeypatch):
+    """With eta = 1 the blend wF*eta + wS*(1-eta) returns the unflattened
+    face values, so the reconstruction equals the reference's plain PPM
+    interface states (schemes_1d.npz) bit for bit."""
+    g = load("schemes_1d.npz")
+    key = f"hd_ppm_{bc}"
+    monkeypatch.setattr(O, "flattener_eta", lambda wS, *a, **k: np.ones(wS.shape[:-1]))
+    s = O.Scheme(dimension=1, subgrid="ppm", boundary=bc, gamma=1.4, dx=1 / 32, flattener="astrea")
+    d = O.reconstruct(g[key + "_grid"], s, 0)
+    assert np.array_equal(d["wFs"][0], g[key + "_wplus"])
+    assert np.array_equal(d["wFs"][1], g[key + "_wminus"])
+    # and the real flattener changes them on this data
+    monkeypatch.undo()
+    d = O.reconstruct(g[key + "_grid"], s, 0)
+    assert not np.array_equal(d["wFs"][0], g[key + "_wplus"])
+
+
+def test_colella90_smooth_monotone_is_unflattened():
+    """Colella's shock test fires only where the zone is compressive and the
+    pressure jump exceeds delta = .33 of the smaller pressure; smooth
+    monotone data passes neither, so eta = 1 everywhere."""
+    x = (np.arange(64) + .5) / 64
+    w = np.zeros((64, 8))
+    w[:, 0] = 1 + .2 * x
+    w[:, 1] = -.5 * x            # compressive everywhere
+    w[:, 4] = 1 + .3 * x ** 2    # relative jumps well below .33
+    for bc in ("edge", "wrap"):
+        assert np.all(O.flattener_eta(w, 0, bc, "colella90") == 1)
+
+
+@pytest.mark.parametrize("bc", ["edge", "wrap"])
+def test_colella90_flattens_only_where_the_shock_test_fires(bc):
+    """One strong compressive pressure jump: eta < 1 appears, and only in
+    cells where the shock test fires or in their neighbour downstream of the
+    pressure gradient (eta_i = min(chi_i, chi_{i - sign(p_{i+1} - p_{i-1})}))."""
+    N = 40
+    w = np.zeros((N, 8))
+    w[:, 0] = 1.
+    w[:, 4] = np.where(np.arange(N) < 20, 10., 1.)
+    w[:, 1] = np.where(np.arange(N) < 20, 1., -1.)     # converging flow at the jump
+    eta = O.flattener_eta(w, 0, bc, "colella90")
+    p, u = O.pad(w[:, 4], bc, 1), O.pad(w[:, 1], bc, 1)
+    dp1 = p[2:] - p[:-2]
+    fires = (u[:-2] - u[2:] > 0) & (np.abs(dp1) / np.minimum(p[2:], p[:-2]) > .33)
+    assert np.any(eta < 1)
+    for i in np.nonzero(eta < 1)[0]:
+        j = (i - int(np.sign(dp1[i]))) % N
+        assert fires[i] or fires[j], i
+
+
+# ---- physical-axis HLLC -----------------------------------------------------
+MIRRORS = [("mirror_ppm_wrap", "ppm", "wrap"), ("mirror_plm_edge", "plm", "edge"), ("mirror_ppm_edge", "ppm", "edge")]
+
+
+@pytest.mark.parametrize("key,subgrid,bc", MIRRORS)
+@pytest.mark.parametrize("parity", [0, 1])
+def test_physical_hllc_y_sweep_equals_reference_x_sweep_of_mirror(key, subgrid, bc, parity):
+    """The y sweep of a physical-axis HLLC on grid G is the reference's own x
+    sweep (HLLC at axis 0, momentum index 1) of G mirrored (axes swapped,
+    v_x <-> v_y), with the momentum components swapped back.  The data are
+    subsonic, so the upwind default never applies."""
+    grid = PINS[key + "_grid"]
+    N = grid.shape[0]
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver="hllc", boundary=bc, gamma=1.4, cfl=.4, dx=1 / N,
+                 compat="physical", parity=parity)
+    fl = O.evolve_space(grid, s)
+    ref = PINS[key + "_mirror_flux_ax0"][..., [0, 2, 1, 3, 4, 5, 6, 7]]
+    assert np.array_equal(fl[1]["flux"], ref)
+    e_ref = float(PINS[key + "_mirror_eigmax_ax0"])
+    assert abs(fl[1]["eigmax"] - e_ref) <= 1e-14 * e_ref
+    # the reference's own axis argument gives a different y flux on this data
+    s_astrea = O.Scheme(dimension=2, subgrid=subgrid, solver="hllc", boundary=bc, gamma=1.4, cfl=.4, dx=1 / N,
+                        parity=parity)
+    assert not np.array_equal(O.evolve_space(grid, s_astrea)[1]["flux"], ref)
+
+
+@pytest.mark.parametrize("case", ["ppm_hllc_wrap"])
+@pytest.mark.parametrize("parity", [0, 1])
+def test_physical_hllc_x_sweep_equals_reference(case, parity):
+    """The x sweep of the physical-axis HLLC is the reference's parity-0 x
+    sweep (axis argument 0) at either parity, on subsonic data (the shear
+    layer: no interface where every wave moves right)."""
+    g = load("space_2d.npz")
+    grid = g[f"{case}_p0_grid"]
+    s = O.Scheme(dimension=2, subgrid=case.split("_")[0], solver="hllc", boundary=case.split("_")[2], gamma=1.4,
+                 dx=1 / 12, compat="physical", parity=parity, eig="lapack")
+    fl = O.evolve_space(grid, s)
+    assert np.array_equal(fl[0]["flux"], g[f"{case}_p0_flux_ax0"])
+
+
+def _hllc_data(wm, wp, ax, gamma=1.4):
+    """The solver's input dict for interface states w- (left) and w+ (right)."""
+    wm, wp = np.atleast_2d(wm).astype(float), np.atleast_2d(wp).astype(float)
+    return {"wFs": (wp, wm), "qFs": (O.prim_to_cons(wp, gamma), O.prim_to_cons(wm, gamma)),
+            "fluxFs": (O.phys_flux(wp, gamma, ax), O.phys_flux(wm, gamma, ax))}
+
+
+def test_physical_hllc_consistent_at_axis_1():
+    """F(w, w) = f(w) in the y sweep: the star momentum is the normal one
+    (index 2).  The reference's index 1 there breaks consistency."""
+    rng = np.random.default_rng(11)
+    w = np.zeros((50, 8))
+    w[:, 0] = rng.uniform(.2, 3, 50)
+    w[:, 1:3] = rng.uniform(-1, 1, (50, 2))
+    w[:, 4] = rng.uniform(.2, 3, 50)
+    d = _hllc_data(w, w, 1)
+    F = O.flux_hllc(1, 1.4, d, mom=2, default_minus=True)
+    np.testing.assert_allclose(F, O.phys_flux(w, 1.4, 1), rtol=1e-13, atol=1e-14)
+    F_ref = O.flux_hllc(1, 1.4, d)
+    assert not np.allclose(F_ref, O.phys_flux(w, 1.4, 1), rtol=1e-6)
+
+
+@pytest.mark.parametrize("ax", [0, 1])
+def test_physical_hllc_upwind_in_supersonic_limits(ax):
+    """All waves moving right (s_L > 0): the flux is F(w-) exactly; all waves
+    moving left (s_R < 0): F(w+) exactly.  The reference returns F(w+) in
+    the first case (solvers.py:133), the downwind state."""
+    wm = np.array([[1., 0, 0, 0, .1, 0, 0, 0]])
+    wp = np.array([[.25, 0, 0, 0, .1, 0, 0, 0]])
+    for sgn in (1., -1.):
+        a, b = np.copy(wm), np.copy(wp)
+        a[0, ax + 1] = b[0, ax + 1] = 3. * sgn
+        d = _hllc_data(a, b, ax)
+        F = O.flux_hllc(ax, 1.4, d, mom=ax + 1, default_minus=True)
+        want = O.phys_flux(a if sgn > 0 else b, 1.4, ax)
+        assert np.array_equal(F, want)
+    a[0, ax + 1] = b[0, ax + 1] = 3.
+    F_ref = O.flux_hllc(ax, 1.4, _hllc_data(a, b, ax))
+    assert np.array_equal(F_ref, O.phys_flux(b, 1.4, ax))
+
+
+def _mirror(q):
+    return np.ascontiguousarray(q.transpose(1, 0, 2)[..., [0, 2, 1, 3, 4, 5, 6, 7]])
+
+
+@pytest.mark.parametrize("subgrid,bc", [("ppm", "wrap"), ("plm", "edge")])
+def test_physical_hllc_run_is_transposition_equivariant(subgrid, bc):
+    """A 2D run with the physical axis started from the mirrored initial
+    state ends in the mirrored final state, bit for bit (the reference's
+    axis argument does not have this symmetry)."""
+    rng = np.random.default_rng(12)
+    N = 24
+    c = (np.arange(N) + .5) / N
+    x, y = np.meshgrid(c, c, indexing="ij")
+    w = np.zeros((N, N, 8))
+    w[..., 0] = 1 + .5 * (x < .4) + .3 * (y > .6) + .05 * rng.random((N, N))
+    w[..., 1] = .3 * np.sin(2 * np.pi * y)
+    w[..., 2] = -.2 * np.cos(2 * np.pi * x)
+    w[..., 4] = 1 + .5 * (x + y < 1) + .05 * rng.random((N, N))
+    kw = dict(dimension=2, subgrid=subgrid, solver="hllc", boundary=bc, gamma=1.4, cfl=.4, dx=1 / N,
+              compat="physical")
+    s = O.Scheme(**kw)
+    q0 = O.initial_average(w, s)
+    a = O.run(q0, s, 1e9, max_steps=6)
+    b = O.run(_mirror(q0), s, 1e9, max_steps=6)
+    assert a.steps == b.steps == 6
+    assert np.array_equal(_mirror(a.grid), b.grid)
+    sa = O.Scheme(**dict(kw, compat="astrea"))
+    assert not np.array_equal(_mirror(O.run(q0, sa, 1e9, max_steps=6).grid), O.run(_mirror(q0), sa, 1e9, max_steps=6).grid)
+
+
+def test_exact_riemann_solver_sod_star_state():
+    """Toro's Table 4.2 test 1 (Sod): p* = 0.30313, u* = 0.92745."""
+    p, u = X.star_state((1., 0., 1.), (.125, 0., .1), 1.4)
+    assert abs(p - .30313) < 5e-6 and abs(u - .92745) < 5e-6
+
+
+def _l1_density(left, right, N, subgrid, t_end, compat="physical"):
+    x = (np.arange(N) + .5) / N
+    w = np.zeros((N, 8))
+    w[:] = [right[0], right[1], 0, 0, right[2], 0, 0, 0]
+    w[x <= .5] = [left[0], left[1], 0, 0, left[2], 0, 0, 0]
+    s = O.Scheme(dimension=1, subgrid=subgrid, solver="hllc", boundary="edge", gamma=1.4, cfl=.5, dx=1 / N,
+                 compat=compat)
+    r = O.run(O.initial_average(w, s), s, t_end)
+    rho = O.to_prim(r.grid, s)[:, 0]
+    exact = X.sample(left, right, 1.4, (x - .5) / t_end)[:, 0]
+    return np.mean(np.abs(rho - exact)), r
+
+
+@pytest.mark.parametrize("subgrid", ["plm", "ppm"])
+def test_physical_hllc_converges_on_toro_test_1(subgrid):
+    """Toro's test 1 (rho, u, p) = (1, .75, 1) | (.125, 0, .1) to t = .2: the
+    L1 density error against the exact solution falls with the resolution
+    at the rate of a scheme that captures a shock and a contact."""
+    errs = [_l1_density((1., .75, 1.), (.125, 0., .1), N, subgrid, .2)[0] for N in (64, 128, 256)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert errs[-1] < 6e-3 and np.all(rates > .6), (errs, rates)
+
+
+def test_physical_hllc_upwind_default_advects_supersonic_contact():
+    """A density contact advected at Mach ~ 5 to the right (every wave moves
+    right, s_L > 0 at every interface): with the upwind default the run
+    converges to the translated profile; the reference's downwind default
+    cannot hold the state finite."""
+    left, right = (1., 2., .1), (.25, 2., .1)
+    errs = [_l1_density(left, right, N, "plm", .15)[0] for N in (64, 128, 256)]
+    assert errs[-1] < errs[1] < errs[0] and errs[-1] < .02, errs
+    e_astrea, r = _l1_density(left, right, 64, "plm", .15, compat="astrea")
+    assert not np.all(np.isfinite(r.grid)) or not e_astrea < 10 * errs[0], e_astrea
+
+
+# ---- physical-axis HLLD -----------------------------------------------------
+def test_physical_hlld_is_the_reference_call_at_the_physical_axis():
+    """At parity 0 the reference's axis argument is the physical axis, so
+    the physical-axis HLLD equals the reference there (space_2d p0); at
+    parity 1 the physical mode makes the same calls, so its fluxes equal the
+    reference's parity-0 fluxes (the reference's own parity-1 fluxes differ)."""
+    g = load("space_2d.npz")
+    for parity in (0, 1):
+        s = O.Scheme(dimension=2, subgrid="weno5", solver="hlld", boundary="wrap", gamma=5 / 3, dx=1 / 12,
+                     magnetic_2d=True, eig="lapack", parity=parity, compat="physical")
+        fl = O.evolve_space(g["weno5_hlld_wrap_p0_grid"], s)
+        for ax in (0, 1):
+            assert np.array_equal(fl[ax]["flux"], g[f"weno5_hlld_wrap_p0_flux_ax{ax}"]), (parity, ax)
+    assert not np.array_equal(g["weno5_hlld_wrap_p1_flux_ax0"], g["weno5_hlld_wrap_p0_flux_ax0"])
