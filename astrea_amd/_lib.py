@@ -26,7 +26,8 @@ EXPORTS = ["ah_version", "ah_status_str", "ah_plan_create", "ah_plan_destroy", "
            "ah_plan_workspace_size", "ah_plan_bind", "ah_plan_reset", "ah_plan_step", "ah_plan_status",
            "ah_evolve_space", "ah_compute_L", "ah_plan_profile", "ah_plan_profile_read", "ah_plan_step_graph",
            "ah_plan_compact", "ah_plan_set_blocks", "ah_plan_time_export", "ah_plan_time_import",
-           "ah_plan_init_ic", "ah_plan_snapshot", "ah_plan_set_capture", "ah_plan_status_async"]
+           "ah_plan_init_ic", "ah_plan_snapshot", "ah_plan_set_capture", "ah_plan_status_async",
+           "ah_evolve_space_ct", "ah_compute_L_ct", "ah_evolve_time"]
 AH_IC_RIEMANN_1D, AH_IC_QUADRANTS_2D = 0, 1
 AH_CAPTURE_OFF, AH_CAPTURE_EVERY_STEP, AH_CAPTURE_CHECKPOINTS = 0, 1, 2
 KERNEL_KINDS = ["sweep1d", "recon_ax0", "flux_ax0", "recon_ax1", "flux_ax1", "combine", "dt", "advance"]
@@ -93,6 +94,9 @@ def load():
     lib.ah_plan_status_async.argtypes = [vp, vp, vp]
     lib.ah_evolve_space.argtypes = [vp, i, i, vp, vp, vp, P(d), vp]
     lib.ah_compute_L.argtypes = [vp, i, i, vp, vp, vp, vp]
+    lib.ah_evolve_space_ct.argtypes = [vp, i, i, vp, vp, vp, P(d), vp, vp, vp, vp]
+    lib.ah_compute_L_ct.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
+    lib.ah_evolve_time.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, d, vp]
     lib.ah_plan_profile.argtypes = [vp, i]
     lib.ah_plan_step_graph.argtypes = [vp, i, i, vp]
     lib.ah_plan_compact.argtypes = [vp, P(i), vp]

@@ -28,7 +28,8 @@ def plan_for(sim_variables, grids, precision=None):
                              max_steps=sv.max_steps))
     mask = infer_varmask(grids, dims, sv.magnetic_2d)
     return Plan(dims, sv.subgrid, sv.solver, sv.timestep, problems, limiter=sv.limiter, flattener=sv.flattener,
-                precision=precision or sv.precision_mode, varmask=mask, magnetic_2d=sv.magnetic_2d)
+                precision=precision or sv.precision_mode, varmask=mask, magnetic_2d=sv.magnetic_2d,
+                solver_category=sv.solver_category)
 
 
 def run_saving(plan, save, t_end, i=0):

@@ -379,6 +379,12 @@ int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
     size_t need;
     ah_plan_workspace_size(p, &need);
     if (ws_bytes < need) return AH_ERR_NOMEM;
+    // a captured step graph holds the previously bound buffers
+    if (p->graph_exec) {
+        HIPCHK(hipGraphExecDestroy(p->graph_exec));
+        p->graph_exec = nullptr;
+        p->graph_steps = 0;
+    }
     char* c = (char*)ws;
     p->U = state;
     p->regs[0] = state;
@@ -511,9 +517,11 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
     }
 }
 
-static void launch_combine(ah_plan* p, int op, hipStream_t s) {
+// stored_dt: combine with the dt stored in the time state (constrained
+// transport stores it after stage 0; ah_evolve_time is handed it)
+static void launch_combine(ah_plan* p, int op, hipStream_t s, bool stored_dt = false) {
     CombineArgs c;
-    c.live_dt = p->sch.magnetic_2d ? 0 : 1;
+    c.live_dt = (p->sch.magnetic_2d || stored_dt) ? 0 : 1;
     c.dims = p->sch.dims;
     for (int r = 0; r < 9; ++r) c.regs[r] = p->regs[r];
     c.L = p->Lbuf;
@@ -612,6 +620,113 @@ static void launch_step(ah_plan* p, hipStream_t s) {
         hipLaunchKernelGGL(capture_commit_kernel, dim3(1), dim3(64), 0, s, p->cap);
     }
 }
+
+}  // extern "C"
+
+namespace {
+// The time state of a single-problem plan replaced for a parity entry point
+// (active, no failure, the given parity and dt) and put back afterwards.
+struct ProbeState {
+    ah_plan* p = nullptr;
+    hipStream_t s = nullptr;
+    ProbState saved;
+    int begin(ah_plan* plan, int parity, double dt, hipStream_t stream) {
+        p = plan;
+        s = stream;
+        HIPCHK(hipMemcpyAsync(&saved, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        ProbState tmp = saved;
+        tmp.active = 1;
+        tmp.fail_flag = 0;
+        tmp.eig[0] = tmp.eig[1] = 0ull;
+        tmp.parity = parity;
+        tmp.dt = dt;
+        HIPCHK(hipMemcpyAsync(p->d_ps, &tmp, sizeof(ProbState), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return AH_OK;
+    }
+    // `after` receives the state the launches left (eigmax bits) before it is restored
+    int end(ProbState* after = nullptr) {
+        if (after) {
+            HIPCHK(hipMemcpyAsync(after, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        HIPCHK(hipMemcpyAsync(p->d_ps, &saved, sizeof(ProbState), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return AH_OK;
+    }
+};
+
+int slot_of(const ah_plan* p, int comp) {
+    for (int v = 0; v < p->nv; ++v)
+        if (p->varidx[v] == comp) return v;
+    return -1;
+}
+
+// the field components the stage-0 reassignment reads (evolvers.py:73-76):
+// B_x of the x sweep's face values and B_y of the y sweep's, in the state's
+// (x, y) cell order; the sweep-frame arrays are [t][s]
+__global__ void ct_faces_kernel(const double* Fx, const double* Fy, double* f0, double* f1, long long plane, int n0,
+                                int n1, int s5, int s6) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= n0 * n1) return;
+    const int i = cell / n1, j = cell % n1;
+    f0[cell] = Fx[s5 * plane + (long long)j * n0 + i];
+    f1[cell] = Fy[s6 * plane + (long long)i * n1 + j];
+}
+
+// B_x, B_y of the state := the given face values; bsave keeps the previous
+// ones (the failure path of copyB_kernel restores them)
+__global__ void ct_assign_kernel(double* U, double* bsave, const double* f0, const double* f1, long long plane,
+                                 int ncell, int s5, int s6) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    bsave[cell] = U[s5 * plane + cell];
+    bsave[plane + cell] = U[s6 * plane + cell];
+    U[s5 * plane + cell] = f0[cell];
+    U[s6 * plane + cell] = f1[cell];
+}
+
+void launch_fluxdiff(ah_plan* p, const double* flux0, const double* flux1, double* L, hipStream_t s) {
+    const ah_problem& q = p->probs[0];
+    const int n1 = p->sch.dims == 1 ? 1 : q.n1;
+    const long long ncell = (long long)q.n0 * n1;
+    const long long fp0 = (long long)(q.n0 + 1) * n1, fp1 = (long long)(n1 + 1) * q.n0;
+    hipLaunchKernelGGL(fluxdiff_kernel, dim3((unsigned)std::min<long long>((ncell + 255) / 256, 4096)), dim3(256), 0,
+                       s, flux0, p->sch.dims == 2 ? flux1 : flux0, L, q.n0, n1, p->sch.dims, p->nv, q.dx, p->total,
+                       fp0, fp1);
+}
+
+// evolve_space of problem 0 (evolvers.py:12-34): fluxes and eigmax per axis;
+// with constrained transport also the corner EMF and the face-averaged field
+int evolve_space_impl(ah_plan* p, int parity, const double* grid, double* flux0, double* flux1, double* eigmax,
+                      double* corner, double* face0, double* face1, hipStream_t s) {
+    ProbeState probe;
+    int rc = probe.begin(p, parity, 0.0, s);
+    if (rc != AH_OK) return rc;
+    launch_space(p, grid, p->Lbuf, 1, parity, flux0, flux1, s);
+    if (corner) {
+        MhdArgs m = mhd_args(p);
+        p->ks.trecon[0](m, blocks_for(p, p->tl[6]), s);
+        p->ks.trecon[1](m, blocks_for(p, p->tl[7]), s);
+        p->ks.corner(m, blocks_for(p, p->tl[7]), s);
+        const ah_problem& q = p->probs[0];
+        const int ncell = q.n0 * q.n1;
+        HIPCHK(hipMemcpyAsync(corner, p->e3, (size_t)ncell * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(ct_faces_kernel, dim3((ncell + 255) / 256), dim3(256), 0, s, (const double*)p->wF[0],
+                           (const double*)p->wF[1], face0, face1, p->total, q.n0, q.n1, slot_of(p, 5), slot_of(p, 6));
+    }
+    HIPCHK(hipGetLastError());
+    ProbState after;
+    rc = probe.end(&after);
+    if (rc != AH_OK) return rc;
+    memcpy(&eigmax[0], &after.eig[0], sizeof(double));
+    if (p->sch.dims == 2) memcpy(&eigmax[1], &after.eig[1], sizeof(double));
+    return AH_OK;
+}
+}  // namespace
+
+extern "C" {
 
 int ah_plan_step(ah_plan* p, int nsteps, void* stream) {
     if (!p || nsteps < 0) return AH_ERR_INVALID;
@@ -840,38 +955,100 @@ int ah_evolve_space(ah_plan* p, int prob, int parity, const double* grid, double
     if (!p || prob != 0 || p->nprob != 1 || !grid || !flux0 || (p->sch.dims == 2 && !flux1) || !eigmax)
         return AH_ERR_INVALID;
     if (!p->bound) return AH_ERR_STATE;
-    hipStream_t s = (hipStream_t)stream;
-    ProbState saved, tmp;
-    HIPCHK(hipMemcpyAsync(&saved, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    tmp = saved;
-    tmp.active = 1;
-    tmp.fail_flag = 0;
-    tmp.eig[0] = tmp.eig[1] = 0ull;
-    tmp.parity = parity;
-    HIPCHK(hipMemcpyAsync(p->d_ps, &tmp, sizeof(ProbState), hipMemcpyHostToDevice, s));
-    launch_space(p, grid, p->Lbuf, 1, parity, flux0, flux1, s);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&tmp, p->d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    memcpy(&eigmax[0], &tmp.eig[0], sizeof(double));
-    if (p->sch.dims == 2) memcpy(&eigmax[1], &tmp.eig[1], sizeof(double));
-    HIPCHK(hipMemcpyAsync(p->d_ps, &saved, sizeof(ProbState), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
-    return AH_OK;
+    return evolve_space_impl(p, parity, grid, flux0, flux1, eigmax, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int ah_evolve_space_ct(ah_plan* p, int prob, int parity, const double* grid, double* flux0, double* flux1,
+                       double* eigmax, double* corner, double* face0, double* face1, void* stream) {
+    if (!p || prob != 0 || p->nprob != 1 || !grid || !flux0 || !flux1 || !eigmax || !corner || !face0 || !face1)
+        return AH_ERR_INVALID;
+    if (!p->sch.magnetic_2d) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    return evolve_space_impl(p, parity, grid, flux0, flux1, eigmax, corner, face0, face1, (hipStream_t)stream);
 }
 
 int ah_compute_L(ah_plan* p, int prob, int parity, const double* flux0, const double* flux1, double* L, void* stream) {
     (void)parity;
     if (!p || prob != 0 || p->nprob != 1 || !flux0 || !L || (p->sch.dims == 2 && !flux1)) return AH_ERR_INVALID;
-    const ah_problem& q = p->probs[0];
-    const int n1 = p->sch.dims == 1 ? 1 : q.n1;
-    const long long ncell = (long long)q.n0 * n1;
-    const long long fp0 = (long long)(q.n0 + 1) * n1, fp1 = (long long)(n1 + 1) * q.n0;
-    hipLaunchKernelGGL(fluxdiff_kernel, dim3((unsigned)std::min<long long>((ncell + 255) / 256, 4096)), dim3(256), 0,
-                       (hipStream_t)stream, flux0, flux1, L, q.n0, n1, p->sch.dims, p->nv, q.dx, p->total, fp0, fp1);
+    // constrained transport overwrites the field rows with EMF differences
+    // (evolvers.py:51-58), which need the corner EMF: ah_compute_L_ct
+    if (p->sch.magnetic_2d) return AH_ERR_UNSUPPORTED;
+    launch_fluxdiff(p, flux0, flux1, L, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return AH_OK;
 }
+
+int ah_compute_L_ct(ah_plan* p, int prob, int parity, const double* flux0, const double* flux1, const double* corner,
+                    double* L, void* stream) {
+    if (!p || prob != 0 || p->nprob != 1 || !flux0 || !flux1 || !corner || !L) return AH_ERR_INVALID;
+    if (!p->sch.magnetic_2d) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    ProbeState probe;
+    int rc = probe.begin(p, parity, 0.0, s);
+    if (rc != AH_OK) return rc;
+    launch_fluxdiff(p, flux0, flux1, L, s);
+    MhdArgs m = mhd_args(p);
+    m.e3 = const_cast<double*>(corner);
+    m.L = L;
+    p->ks.magL(m, blocks_for(p, p->tl[5]), s);
+    HIPCHK(hipGetLastError());
+    return probe.end();
+}
+
+int ah_evolve_time(ah_plan* p, int prob, int parity, double* grid, const double* flux0, const double* flux1,
+                   const double* corner, const double* face0, const double* face1, double dt, void* stream) {
+    if (!p || prob != 0 || p->nprob != 1 || !grid || !flux0 || (p->sch.dims == 2 && !flux1)) return AH_ERR_INVALID;
+    const bool mhd = p->sch.magnetic_2d != 0;
+    if (mhd && (!corner || !face0 || !face1)) return AH_ERR_INVALID;
+    if (!p->bound) return AH_ERR_STATE;
+    hipStream_t s = (hipStream_t)stream;
+    ProbeState probe;
+    int rc = probe.begin(p, parity, dt, s);
+    if (rc != AH_OK) return rc;
+    // the stages update `grid` (the stage registers stay the plan's own)
+    double* U_bound = p->U;
+    p->U = grid;
+    p->regs[0] = grid;
+    const ah_problem& q = p->probs[0];
+    const int ncell = q.n0 * q.n1;
+    // L0 from the given fluxes (evolvers.py:62)
+    launch_fluxdiff(p, flux0, flux1, p->Lbuf, s);
+    if (mhd) {
+        MhdArgs m = mhd_args(p);
+        m.e3 = const_cast<double*>(corner);
+        p->ks.magL(m, blocks_for(p, p->tl[5]), s);
+        // B_x, B_y of the grid := the sweeps' face values, in place (evolvers.py:73-76)
+        hipLaunchKernelGGL(ct_assign_kernel, dim3((ncell + 255) / 256), dim3(256), 0, s, grid, p->bsave, face0, face1,
+                           p->total, ncell, slot_of(p, 5), slot_of(p, 6));
+    }
+    for (size_t st = 0; st < p->prog.size(); ++st) {
+        const Stage& S = p->prog[st];
+        if (st > 0) {
+            launch_space(p, p->regs[S.in], p->Lbuf, 0, -1, nullptr, nullptr, s);
+            if (mhd) {
+                MhdArgs m = mhd_args(p);
+                p->ks.trecon[0](m, blocks_for(p, p->tl[6]), s);
+                p->ks.trecon[1](m, blocks_for(p, p->tl[7]), s);
+                p->ks.corner(m, blocks_for(p, p->tl[7]), s);
+                p->ks.magL(m, blocks_for(p, p->tl[5]), s);
+            }
+        }
+        launch_combine(p, S.op, s, true);
+        if (mhd) {
+            // refine_grid after every stage (evolvers.py:63-67)
+            MhdArgs m = mhd_args(p);
+            double* K = p->regs[target_reg(S.op)];
+            m.K = K;
+            p->ks.refine(m, blocks_for(p, p->tl[7]), s);
+            p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
+        }
+    }
+    p->U = U_bound;
+    p->regs[0] = U_bound;
+    HIPCHK(hipGetLastError());
+    return probe.end();
+}
+
 
 }  // extern "C"

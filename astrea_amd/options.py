@@ -11,6 +11,8 @@ colella90), ``precision_mode`` (fp64 | split) and ``max_steps``.
 import itertools
 from collections import namedtuple
 
+import numpy as np
+
 from . import _lib as L
 from . import configs
 
@@ -105,7 +107,12 @@ def handle_variables(params, boundary=None, dx=None, t_end=None):
     out.setdefault("checkpoints", 1)
     out.setdefault("max_steps", -1)
     if out["gamma"] == 1:
-        out["gamma"] = 1 + 2.220446049250313e-16
+        # generic.py:193-194: the machine epsilon of the parameters' own precision
+        out["gamma"] = out["gamma"] + float(np.finfo(params.get("precision", "float64")).eps)
+    if out["precision"] not in ("float", "float64"):
+        # constructor.py:24 builds the reference's grid in this dtype; the device
+        # update computes in FP64 (or the split mode of precision_mode) whatever it says
+        print(f"PRECISION {out['precision']}: the device update runs in float64..")
     cc = _category("config", out["config"])
     out["config_category"] = cc
     out["solver_category"] = _category("solver", out["solver"])
@@ -113,9 +120,11 @@ def handle_variables(params, boundary=None, dx=None, t_end=None):
     if "2D" in (cc or "") and out["dimension"] != 2:
         out["dimension"] = 2
     if out["solver_category"] == "hll" and out["solver"].endswith("c") and cc in ("magnetic",):
+        # generic.py:266-269 resets the solver name only: solver_category stays
+        # "hll" (set at generic.py:241), and solvers.py:15-19 dispatches on the
+        # category first, so the name "lf" (not ending in "d") still runs HLLC
         print("HLLC solver does not work with magnetic fields present..")
         out["solver"] = DEFAULTS["solver"]
-        out["solver_category"] = "lax"
     d = out["dimension"]
     perms = [a for a in itertools.permutations(range(d + 1)) if a[-1] == d]
     out["permutations"] = {i: a for i, a in enumerate(perms)}

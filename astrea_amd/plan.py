@@ -34,7 +34,8 @@ def infer_varmask(grids, dims, magnetic):
 
 class Plan:
     def __init__(self, dims, subgrid, solver, timestep, problems, *, limiter="minmod", flattener="off",
-                 precision="fp64", varmask=L.AH_VARS_H1, compat="astrea", magnetic_2d=False, device="cuda:0"):
+                 precision="fp64", varmask=L.AH_VARS_H1, compat="astrea", magnetic_2d=False, device="cuda:0",
+                 solver_category=None):
         self.lib = L.load()
         self.dims = dims
         self.device = torch.device(device)
@@ -44,7 +45,7 @@ class Plan:
         sch.recon = O.recon_of(subgrid)
         sch.limiter = O.LIMITERS[limiter]
         sch.flattener = O.FLATTENERS[flattener]
-        sch.solver = O.solver_of(solver)
+        sch.solver = O.solver_of(solver, solver_category)
         sch.rk = O.rk_of(timestep)
         sch.precision = L.AH_SPLIT if precision == "split" else L.AH_FP64
         sch.varmask = varmask
@@ -297,9 +298,16 @@ class Plan:
         return {k: (ms[i], cnt[i]) for i, k in enumerate(L.KERNEL_KINDS) if cnt[i]}
 
     # ---- parity entry points (single-problem plans) ----------------------------
+    def _ptr(self, t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
     def evolve_space(self, parity=0, state=None):
-        """Interface fluxes per physical axis in the reference's sweep-frame
-        layout ((n_a + 1, n_other, 8) in 2D) and eigmax per axis."""
+        """evolvers.evolve_space: interface fluxes per physical axis in the
+        reference's sweep-frame layout ((n_a + 1, n_other, 8) in 2D) and
+        eigmax per axis; with constrained transport also 'mag_corner' (the
+        corner EMF in the state's (x, y) order, as the reference attaches it
+        to the axis-1 sweep; the axis-0 entry holds its transpose) and
+        'face_avg' (the sweep's face-valued B component, (x, y) order)."""
         assert self.nprob == 1
         p = self.problems[0]
         X = self.state if state is None else state
@@ -307,9 +315,16 @@ class Plan:
         f0 = torch.zeros((self.nv, (n0 + 1) * n1), dtype=torch.float64, device=self.device)
         f1 = torch.zeros((self.nv, (n1 + 1) * n0), dtype=torch.float64, device=self.device)
         eig = (ctypes.c_double * 2)()
-        L.check(self.lib.ah_evolve_space(self.h, 0, int(parity), ctypes.c_void_p(X.data_ptr()),
-                                         ctypes.c_void_p(f0.data_ptr()), ctypes.c_void_p(f1.data_ptr()), eig,
-                                         self._stream()), "ah_evolve_space")
+        ct = bool(self.scheme.magnetic_2d)
+        if ct:
+            corner = torch.zeros((n0, n1), dtype=torch.float64, device=self.device)
+            faces = torch.zeros((2, n0, n1), dtype=torch.float64, device=self.device)
+            L.check(self.lib.ah_evolve_space_ct(self.h, 0, int(parity), self._ptr(X), self._ptr(f0), self._ptr(f1),
+                                                eig, self._ptr(corner), self._ptr(faces[0]), self._ptr(faces[1]),
+                                                self._stream()), "ah_evolve_space_ct")
+        else:
+            L.check(self.lib.ah_evolve_space(self.h, 0, int(parity), self._ptr(X), self._ptr(f0), self._ptr(f1),
+                                             eig, self._stream()), "ah_evolve_space")
         out = {}
         for ax, (f, ns, nt) in enumerate([(f0, n0, n1), (f1, n1, n0)][:self.dims]):
             h = f.cpu().numpy()
@@ -318,16 +333,44 @@ class Plan:
                 g[:, c] = h[v]
             shape = (ns + 1,) if self.dims == 1 else (ns + 1, nt)
             out[ax] = {"flux": g.reshape(shape + (8,)), "eigmax": eig[ax], "raw": f}
+            if ct:
+                e3 = corner.cpu().numpy()
+                out[ax]["mag_corner"] = e3 if ax == 1 else np.ascontiguousarray(e3.T)
+                out[ax]["face_avg"] = faces[ax].cpu().numpy()
+                out[ax]["raw_corner"], out[ax]["raw_face"] = corner, faces[ax]
         return out
 
     def compute_L(self, fluxes, parity=0):
+        """evolvers.compute_L (with the EMF rows of constrained transport)
+        in the packed state layout."""
         L_ = torch.zeros_like(self.state)
         f0 = fluxes[0]["raw"]
         f1 = fluxes[1]["raw"] if self.dims == 2 else f0
-        L.check(self.lib.ah_compute_L(self.h, 0, int(parity), ctypes.c_void_p(f0.data_ptr()),
-                                      ctypes.c_void_p(f1.data_ptr()), ctypes.c_void_p(L_.data_ptr()),
-                                      self._stream()), "ah_compute_L")
+        if self.scheme.magnetic_2d:
+            L.check(self.lib.ah_compute_L_ct(self.h, 0, int(parity), self._ptr(f0), self._ptr(f1),
+                                             self._ptr(fluxes[1]["raw_corner"]), self._ptr(L_), self._stream()),
+                    "ah_compute_L_ct")
+        else:
+            L.check(self.lib.ah_compute_L(self.h, 0, int(parity), self._ptr(f0), self._ptr(f1), self._ptr(L_),
+                                          self._stream()), "ah_compute_L")
         return L_
+
+    def evolve_time(self, fluxes, dt, parity=0, state=None):
+        """evolvers.evolve_time on ``state`` (default: the plan's state) in
+        place: L0 from ``fluxes`` (as evolve_space returns them), the field
+        reassignment and refinement of constrained transport, the plan's
+        Runge-Kutta stages.  Returns the updated state tensor."""
+        X = self.state if state is None else state
+        f0 = fluxes[0]["raw"]
+        f1 = fluxes[1]["raw"] if self.dims == 2 else f0
+        ct = bool(self.scheme.magnetic_2d)
+        corner = fluxes[1]["raw_corner"] if ct else None
+        face0 = fluxes[0]["raw_face"] if ct else None
+        face1 = fluxes[1]["raw_face"] if ct else None
+        L.check(self.lib.ah_evolve_time(self.h, 0, int(parity), self._ptr(X), self._ptr(f0), self._ptr(f1),
+                                        self._ptr(corner), self._ptr(face0), self._ptr(face1), float(dt),
+                                        self._stream()), "ah_evolve_time")
+        return X
 
 
 class PendingStatus:

@@ -142,7 +142,8 @@ int ah_plan_layout(const ah_plan* plan, long long* total_cells, int* nvars, int*
                    long long* problem_offsets);
 
 /* Caller-provided device workspace (stage registers, interface states,
- * reductions).  No allocation happens in ah_plan_step. */
+ * reductions).  No allocation happens in ah_plan_step.  Binding again (new
+ * state or workspace) drops a step graph captured with the old buffers. */
 int ah_plan_workspace_size(const ah_plan* plan, size_t* bytes);
 int ah_plan_bind(ah_plan* plan, double* state, void* workspace, size_t workspace_bytes);
 
@@ -250,9 +251,38 @@ int ah_evolve_space(ah_plan* plan, int prob, int parity, const double* grid,
                     double* flux0, double* flux1, double* eigmax, void* stream);
 
 /* ah_compute_L replaces compute_L (evolvers.py:41-60): L = -sum_a dF_a/dx for
- * problem `prob`, from fluxes laid out as ah_evolve_space writes them. */
+ * problem `prob`, from fluxes laid out as ah_evolve_space writes them (`L`:
+ * the packed layout of the state).  Plans with constrained transport return
+ * AH_ERR_UNSUPPORTED: their field rows need the corner EMF, ah_compute_L_ct. */
 int ah_compute_L(ah_plan* plan, int prob, int parity, const double* flux0, const double* flux1,
                  double* L, void* stream);
+
+/* Constrained transport (magnetic_2d plans).  ah_evolve_space_ct is
+ * ah_evolve_space plus the EMF attachment of evolvers.py:26-32: `corner`
+ * receives the corner EMF e3U (mag_field.py:125-187; n0*n1 doubles in the
+ * state's (x, y) order, the reference's fluxes[(1,0,2)]['mag_corner'];
+ * fluxes[(0,1,2)]['mag_corner'] is its transpose), `face0` / `face1` the
+ * components of fluxes[...]['face_avg'] that evolve_time reads: B_x of the
+ * x sweep's face values and B_y of the y sweep's, in the state's (x, y)
+ * order.  ah_compute_L_ct is compute_L with the EMF-difference rows of B_x,
+ * B_y (evolvers.py:51-58) from that corner EMF. */
+int ah_evolve_space_ct(ah_plan* plan, int prob, int parity, const double* grid, double* flux0, double* flux1,
+                       double* eigmax, double* corner, double* face0, double* face1, void* stream);
+int ah_compute_L_ct(ah_plan* plan, int prob, int parity, const double* flux0, const double* flux1,
+                    const double* corner, double* L, void* stream);
+
+/* ah_evolve_time replaces evolve_time(grid, fluxes, dt, sim_variables)
+ * (evolvers.py:38-206) for problem `prob` of a single-problem plan at step
+ * parity `parity`: L0 from the given fluxes (and, with constrained
+ * transport, `corner`), the in-place reassignment of B_x, B_y to the face
+ * values `face0`, `face1` (evolvers.py:73-76), the plan's Runge-Kutta stages
+ * with evolve_space at each later stage and, with constrained transport, the
+ * inverse reconstruction of the field after every stage (evolvers.py:63-67).
+ * `grid` (the packed state layout) is overwritten with the result; the
+ * plan's own state and time state are left as they were.  `corner`, `face0`
+ * and `face1` may be null for plans without constrained transport. */
+int ah_evolve_time(ah_plan* plan, int prob, int parity, double* grid, const double* flux0, const double* flux1,
+                   const double* corner, const double* face0, const double* face1, double dt, void* stream);
 
 /* Kernel timing with HIP events inside ah_plan_step (no reference
  * counterpart: the reference's only timing is perf_counter around a whole run,

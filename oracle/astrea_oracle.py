@@ -80,6 +80,7 @@ class Scheme:
     eig: str = "closed"           # closed | lapack
     parity: int = 0               # step parity: sweep order flips every step
     compat: str = "astrea"        # astrea: reference quirks | physical: solver axis = sweep axis
+    category: str = ""            # solver_category when it differs from the name's (generic.py:241,266-269)
 
     @property
     def high_order(self):
@@ -87,8 +88,12 @@ class Scheme:
 
     @property
     def solver_category(self):
-        return {"llf": "lax", "lf": "lax", "lw": "lax", "hllc": "hll", "hlld": "hll",
-                "os": "complete", "es": "complete"}[self.solver]
+        """generic.py:241 derives the category from the solver name; a
+        magnetic configuration with HLLC then has its name reset to 'lf' while
+        the category stays 'hll' (generic.py:266-269), which ``category``
+        carries."""
+        return self.category or {"llf": "lax", "lf": "lax", "lw": "lax", "hllc": "hll", "hlld": "hll",
+                                  "os": "complete", "es": "complete"}[self.solver]
 
     def order(self):
         """Sweep order: physical axes in the order the reference iterates its

@@ -73,3 +73,20 @@ def test_invalid_values_revert_to_defaults():
     assert sv.subgrid == "pcm" and sv.dimension == 2 and sv.cells == 64
     sv = O.handle_variables({"config": "brio-wu", "solver": "hllc"})
     assert sv.solver == "lf"          # generic.py:266-269
+    # ... but the category stays "hll" (generic.py:241), and solvers.py:15-19
+    # dispatches on it first: the reference still runs HLLC
+    assert sv.solver_category == "hll"
+    assert O.solver_of(sv.solver, sv.solver_category) == L.AH_HLLC
+    sv = O.handle_variables({"config": "brio-wu", "solver": "hlld"})
+    assert sv.solver == "hlld" and O.solver_of(sv.solver, sv.solver_category) == L.AH_HLLD
+    sv = O.handle_variables({"config": "sod", "solver": "hllc"})
+    assert sv.solver == "hllc" and O.solver_of(sv.solver, sv.solver_category) == L.AH_HLLC
+
+
+def test_gamma_one_takes_the_epsilon_of_the_stated_precision():
+    """generic.py:193-194: gamma == 1 is bumped by np.finfo(precision).eps."""
+    import numpy as np
+    assert O.handle_variables({"gamma": 1.0, "precision": "float32"}).gamma == 1 + float(np.finfo("float32").eps)
+    assert O.handle_variables({"gamma": 1.0}).gamma == 1 + float(np.finfo("float64").eps)
+    assert O.handle_variables({"gamma": 1.0, "precision": "float16"}).gamma == 1 + float(np.finfo("float16").eps)
+    assert O.handle_variables({"gamma": 1}).gamma == 1.4      # an int is not a valid gamma (generic.py:188-192)

@@ -2,6 +2,7 @@
 reference's golden trajectories.  Every case prints its name before touching
 the GPU."""
 import sys
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -757,8 +758,11 @@ def test_graph_chunks_after_eager_steps_equal_eager():
 
 @pytest.mark.parametrize("compat", ["astrea", "physical"])
 def test_batch_vs_oracle(compat):
-    """Every non-failing problem of a batch against the oracle, in both
-    compatibility modes."""
+    """Every problem of a batch against the oracle, in both compatibility
+    modes: a completed problem ends in the oracle's state after the oracle's
+    step count; a problem the device flags as failed after n good steps is
+    one on which the oracle with the reference's LAPACK eigenvalues raises in
+    step n + 1, and the device keeps the oracle's state after n steps."""
     from astrea_amd.plan import Plan
     from astrea_amd import batch
     grids, probs = _ensemble(6, 5, 32, 128)
@@ -769,15 +773,26 @@ def test_batch_vs_oracle(compat):
     st, _, _ = batch.run(plan, chunk=8)
     outs = plan.unpack()
     plan.close()
+    failed = 0
     for i in range(len(grids)):
         p = probs[i]
         s = O.Scheme(dimension=1, subgrid="ppm", solver="hllc", boundary="edge", gamma=1.4, cfl=.5, dx=p["dx"],
                      compat=compat)
-        ref = O.run(grids[i], s, p["t_end"])
         if st[i]["failed"]:
+            n_ok = st[i]["steps"]
+            _say(f"[gpu]   problem {i} failed after {n_ok} steps")
+            lap = replace(s, eig="lapack")
+            with pytest.raises(np.linalg.LinAlgError):
+                O.run(grids[i], lap, p["t_end"], 1, n_ok + 1)
+            ref = O.run(grids[i], s, p["t_end"], 1, n_ok)
+            assert ref.steps == n_ok
+            assert np.max(rel_l1(outs[i], ref.grid)) <= 1e-12
+            failed += 1
             continue
+        ref = O.run(grids[i], s, p["t_end"])
         assert st[i]["steps"] == ref.steps
         assert np.max(rel_l1(outs[i], ref.grid)) <= 1e-12
+    _say(f"[gpu]   {failed} of {len(grids)} problems failed on the device and in the oracle")
 
 
 def test_served_results_equal_alone():
