@@ -10,7 +10,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libastrea_hip.so")
 
-AH_OK, AH_ERR_INVALID, AH_ERR_UNSUPPORTED, AH_ERR_HIP, AH_ERR_NOMEM, AH_ERR_STATE = 0, -1, -2, -3, -4, -5
+AH_OK, AH_ERR_INVALID, AH_ERR_UNSUPPORTED, AH_ERR_HIP, AH_ERR_NOMEM, AH_ERR_STATE, AH_ERR_TOO_LARGE = 0, -1, -2, -3, -4, -5, -6
+AH_STATIC_SCRATCH_BYTES = 64 << 20
 AH_EDGE, AH_WRAP = 0, 1
 AH_PCM, AH_PLM, AH_PPM, AH_WENO3, AH_WENO5, AH_WENO7 = range(6)
 AH_MINMOD, AH_VANLEER, AH_OSPRE, AH_VANALBADA1, AH_KOREN, AH_SUPERBEE = range(6)
@@ -27,7 +28,7 @@ EXPORTS = ["ah_version", "ah_status_str", "ah_plan_create", "ah_plan_destroy", "
            "ah_evolve_space", "ah_compute_L", "ah_plan_profile", "ah_plan_profile_read", "ah_plan_step_graph",
            "ah_plan_compact", "ah_plan_set_blocks", "ah_plan_time_export", "ah_plan_time_import",
            "ah_plan_init_ic", "ah_plan_snapshot", "ah_plan_set_capture", "ah_plan_status_async",
-           "ah_evolve_space_ct", "ah_compute_L_ct", "ah_evolve_time"]
+           "ah_evolve_space_ct", "ah_compute_L_ct", "ah_evolve_time", "ah_evolve_space_static"]
 AH_IC_RIEMANN_1D, AH_IC_QUADRANTS_2D = 0, 1
 AH_CAPTURE_OFF, AH_CAPTURE_EVERY_STEP, AH_CAPTURE_CHECKPOINTS = 0, 1, 2
 KERNEL_KINDS = ["sweep1d", "recon_ax0", "flux_ax0", "recon_ax1", "flux_ax1", "combine", "dt", "advance"]
@@ -97,6 +98,7 @@ def load():
     lib.ah_evolve_space_ct.argtypes = [vp, i, i, vp, vp, vp, P(d), vp, vp, vp, vp]
     lib.ah_compute_L_ct.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
     lib.ah_evolve_time.argtypes = [vp, i, i, vp, vp, vp, vp, vp, vp, d, vp]
+    lib.ah_evolve_space_static.argtypes = [P(AhScheme), P(AhProblem), i, vp, vp, vp, P(d), vp]
     lib.ah_plan_profile.argtypes = [vp, i]
     lib.ah_plan_step_graph.argtypes = [vp, i, i, vp]
     lib.ah_plan_compact.argtypes = [vp, P(i), vp]

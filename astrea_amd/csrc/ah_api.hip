@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -209,6 +210,7 @@ const char* ah_status_str(int status) {
         case AH_ERR_HIP: return "HIP runtime error";
         case AH_ERR_NOMEM: return "workspace too small or allocation failed";
         case AH_ERR_STATE: return "plan not bound or used out of order";
+        case AH_ERR_TOO_LARGE: return "problem exceeds the static scratch array (AH_STATIC_SCRATCH_BYTES)";
     }
     return "unknown status";
 }
@@ -1051,4 +1053,218 @@ int ah_evolve_time(ah_plan* p, int prob, int parity, double* grid, const double*
 }
 
 
+// ============================================================================
+// ah_evolve_space_static: ah_evolve_space for one problem without a plan and
+// without allocating.  The problem tables, the tile lists and the interface-
+// state workspace live in `scratch`, a fixed array of every device the
+// library runs on (held until the process ends).  Calls on one device share
+// that array, so calls that overlap on one device race; each call rebuilds
+// what it reads (tables, tiles, zeroed eigmax) before its sweeps.
+// ============================================================================
+}  // extern "C"
+
+static __device__ __align__(256) unsigned char scratch[AH_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// the device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side name is not a device address)
+int scratch_ptr(unsigned char** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return AH_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return AH_ERR_HIP;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = (unsigned char*)g_scratch_ptr[dev];
+    return AH_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// tile list k -> (s0 = (k % nsb) * ts, t0 = (k / nsb) * tt) of one problem
+struct TileSpec {
+    Tile* dst;
+    int* count;
+    int ns, ts, tt, n;  // n tiles, nsb = ceil(ns / ts)
+};
+
+__global__ void static_init_kernel(ProbDesc* pd, ProbDesc desc, ProbState* ps, int parity, TileSpec t0, TileSpec t1,
+                                   TileSpec t2, TileSpec t3, int nlists) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        *pd = desc;
+        ProbState s;
+        s.t = 0.0;
+        s.dt = 0.0;
+        s.steps = 0;
+        s.idx = 1;
+        s.active = 1;
+        s.failed = 0;
+        s.parity = parity;
+        s.eig[0] = s.eig[1] = 0ull;
+        s.fail_flag = 0;
+        s.ckpt = 0;
+        *ps = s;
+    }
+    const TileSpec specs[4] = {t0, t1, t2, t3};
+    for (int l = 0; l < nlists; ++l) {
+        const TileSpec& t = specs[l];
+        if (k == 0) *t.count = t.n;
+        if (k < t.n) {
+            const int nsb = (t.ns + t.ts - 1) / t.ts;
+            t.dst[k] = Tile{0, (k % nsb) * t.ts, (k / nsb) * t.tt, 0};
+        }
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity, const double* grid, double* flux0,
+                           double* flux1, double* eigmax, void* stream) {
+    if (!sch || !q || !grid || !flux0 || !eigmax || (sch->dims != 1 && sch->dims != 2) ||
+        (sch->dims == 2 && !flux1) || (parity != 0 && parity != 1))
+        return AH_ERR_INVALID;
+    if (sch->magnetic_2d) return AH_ERR_UNSUPPORTED;
+    const int n0 = q->n0, n1 = sch->dims == 1 ? 1 : q->n1;
+    if (n0 <= 0 || n1 <= 0 || q->dx <= 0 || q->checkpoints <= 0 || (q->boundary != AH_EDGE && q->boundary != AH_WRAP))
+        return AH_ERR_INVALID;
+    KernelSet ks;
+    bool ok = sch->precision == AH_FP64
+                  ? (sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &ks)
+                                    : kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &ks))
+                  : (sch->dims == 1 ? kernels_1d_fp32(sch->varmask, sch->recon, sch->solver, &ks)
+                                    : kernels_2d_fp32(sch->varmask, sch->recon, sch->solver, &ks));
+    if (!ok) return AH_ERR_UNSUPPORTED;
+    int nv = 0;
+    for (int c = 0; c < 8; ++c) nv += (sch->varmask >> c) & 1;
+    const long long cells = (long long)n0 * n1;
+    // tile lists: 1D sweep, or recon x/y and flux x/y
+    TileSpec spec[4];
+    int nlists = 0;
+    auto add = [&](int ns, int nt, int ts, int tt) {
+        const int n = ((ns + ts - 1) / ts) * ((nt + tt - 1) / tt);
+        spec[nlists++] = TileSpec{nullptr, nullptr, ns, ts, tt, n};
+    };
+    if (sch->dims == 1) {
+        add(n0, 1, ks.ts1, 1);
+    } else {
+        add(n0, n1, ks.rts[0], ks.rtt[0]);
+        add(n1, n0, ks.rts[1], ks.rtt[1]);
+        add(n0, n1, ks.fts[0], ks.ftt[0]);
+        add(n1, n0, ks.fts[1], ks.ftt[1]);
+    }
+    // scratch layout: tables, tile counts and lists, L, D, wL, wR, normal field
+    const size_t plane8 = align256((size_t)nv * cells * 8), planeR = align256((size_t)nv * cells * ks.real_bytes);
+    size_t need = align256(sizeof(ProbDesc)) + align256(sizeof(ProbState)) + align256(4 * sizeof(int));
+    for (int l = 0; l < nlists; ++l) need += align256((size_t)spec[l].n * sizeof(Tile));
+    need += plane8;  // L
+    if (sch->dims == 2) need += plane8 + 2 * planeR + align256((size_t)cells * ks.real_bytes);
+    if (need > (size_t)AH_STATIC_SCRATCH_BYTES || cells >= (1ll << 31)) return AH_ERR_TOO_LARGE;
+    unsigned char* base = nullptr;
+    int rc = scratch_ptr(&base);
+    if (rc != AH_OK) return rc;
+    unsigned char* c = base;
+    ProbDesc* d_pd = (ProbDesc*)c;
+    c += align256(sizeof(ProbDesc));
+    ProbState* d_ps = (ProbState*)c;
+    c += align256(sizeof(ProbState));
+    int* d_counts = (int*)c;
+    c += align256(4 * sizeof(int));
+    for (int l = 0; l < nlists; ++l) {
+        spec[l].count = d_counts + l;
+        spec[l].dst = (Tile*)c;
+        c += align256((size_t)spec[l].n * sizeof(Tile));
+    }
+    double* Lbuf = (double*)c;
+    c += plane8;
+    double* Dbuf = nullptr;
+    void *wL = nullptr, *wR = nullptr, *bn = nullptr;
+    if (sch->dims == 2) {
+        Dbuf = (double*)c;
+        c += plane8;
+        wL = c;
+        c += planeR;
+        wR = c;
+        c += planeR;
+        bn = c;
+    }
+    ProbDesc desc;
+    memset(&desc, 0, sizeof(desc));
+    desc.n0 = n0;
+    desc.n1 = n1;
+    desc.bc = q->boundary;
+    desc.off = 0;
+    desc.dx = q->dx;
+    desc.gamma = q->gamma;
+    desc.cfl = q->cfl;
+    desc.t_end = q->t_end;
+    desc.chkpt = q->t_end / q->checkpoints;
+    desc.max_steps = q->max_steps;
+    hipStream_t s = (hipStream_t)stream;
+    int maxn = 1;
+    for (int l = 0; l < nlists; ++l) maxn = std::max(maxn, spec[l].n);
+    for (int l = nlists; l < 4; ++l) spec[l] = spec[0];
+    hipLaunchKernelGGL(static_init_kernel, dim3((maxn + 255) / 256), dim3(256), 0, s, d_pd, desc, d_ps, parity, spec[0],
+                       spec[1], spec[2], spec[3], nlists);
+    HIPCHK(hipGetLastError());
+    // the sweeps of launch_space with this problem's tables
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.plane = cells;
+    a.pd = d_pd;
+    a.ps = d_ps;
+    a.limiter = sch->limiter;
+    a.flattener = sch->flattener;
+    a.parity_override = parity;
+    a.compat = sch->compat;
+    a.X = grid;
+    a.stage0 = 1;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    auto blocks = [&](int l) { return std::min(spec[l].n, 32 * cus); };
+    if (sch->dims == 1) {
+        a.out = Lbuf;
+        a.tiles = spec[0].dst;
+        a.ntiles_dev = spec[0].count;
+        a.flux_out = flux0;
+        a.flux_plane = n0 + 1;
+        ks.sweep1d(a, blocks(0), s);
+    } else {
+        a.wL = wL;
+        a.wR = wR;
+        a.bn = sch->solver == AH_HLLD ? bn : nullptr;
+        for (int ax = 0; ax < 2; ++ax) {
+            a.tiles = spec[ax].dst;
+            a.ntiles_dev = spec[ax].count;
+            ks.recon[ax](a, blocks(ax), s);
+            a.tiles = spec[2 + ax].dst;
+            a.ntiles_dev = spec[2 + ax].count;
+            a.last = ax == 1;
+            a.out = ax == 0 ? Dbuf : Lbuf;
+            a.Dprev = Dbuf;
+            a.flux_out = ax == 0 ? flux0 : flux1;
+            a.flux_plane = ax == 0 ? (long long)(n0 + 1) * n1 : (long long)(n1 + 1) * n0;
+            ks.flux[ax](a, blocks(2 + ax), s);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    ProbState after;
+    HIPCHK(hipMemcpyAsync(&after, d_ps, sizeof(ProbState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(&eigmax[0], &after.eig[0], sizeof(double));
+    if (sch->dims == 2) memcpy(&eigmax[1], &after.eig[1], sizeof(double));
+    return AH_OK;
+}
 }  // extern "C"

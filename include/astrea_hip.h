@@ -42,7 +42,8 @@ enum ah_status {
     AH_ERR_UNSUPPORTED = -2, /* option the device path does not implement */
     AH_ERR_HIP = -3,         /* a HIP runtime call failed */
     AH_ERR_NOMEM = -4,       /* workspace too small / allocation failed */
-    AH_ERR_STATE = -5        /* plan not bound, or used out of order */
+    AH_ERR_STATE = -5,       /* plan not bound, or used out of order */
+    AH_ERR_TOO_LARGE = -6    /* problem larger than the static scratch array (ah_evolve_space_static) */
 };
 
 /* options -- same meaning as astrea's option strings (static/.db.json:220-347) */
@@ -283,6 +284,22 @@ int ah_compute_L_ct(ah_plan* plan, int prob, int parity, const double* flux0, co
  * and `face1` may be null for plans without constrained transport. */
 int ah_evolve_time(ah_plan* plan, int prob, int parity, double* grid, const double* flux0, const double* flux1,
                    const double* corner, const double* face0, const double* face1, double dt, void* stream);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  ah_evolve_space for one problem without a plan: the same
+ * sweeps, fluxes and eigmax as ah_evolve_space on a single-problem plan of
+ * (`scheme`, `problem`), with `grid` the packed state of that plan (nvars
+ * planes of n0*n1 doubles).  The problem tables, tile lists and interface-
+ * state workspace live in a fixed device array of AH_STATIC_SCRATCH_BYTES
+ * per device (held until the process ends); nothing is allocated.  A problem
+ * whose workspace exceeds it returns AH_ERR_TOO_LARGE (2D hydrodynamics in
+ * FP64 needs 136 bytes per cell: about 490 000 cells).  Calls on one device
+ * share the array, so calls that overlap on one device race.  Constrained
+ * transport is not supported (AH_ERR_UNSUPPORTED).  Synchronises `stream`. */
+#define AH_STATIC_SCRATCH_BYTES (64ll << 20)
+int ah_evolve_space_static(const ah_scheme* scheme, const ah_problem* problem, int parity, const double* grid,
+                           double* flux0, double* flux1, double* eigmax, void* stream);
 
 /* Kernel timing with HIP events inside ah_plan_step (no reference
  * counterpart: the reference's only timing is perf_counter around a whole run,
