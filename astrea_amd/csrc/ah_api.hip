@@ -218,6 +218,9 @@ const char* ah_status_str(int status) {
 int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_plan** out) {
     if (!sch || !probs || !out || nprob <= 0) return AH_ERR_INVALID;
     if (sch->dims != 1 && sch->dims != 2) return AH_ERR_INVALID;
+    // artificial viscosity: the reference computes it for PPM only (ppm.py:94-95),
+    // and its 2D form cannot run (ppm.py:149-154 adds arrays of different shapes)
+    if (sch->av && (sch->dims != 1 || sch->recon != AH_PPM)) return AH_ERR_UNSUPPORTED;
     if (sch->magnetic_2d && (sch->dims != 2 || !(sch->varmask & 0x60) || sch->rk == AH_SSPRK104 ||
                              sch->flattener != AH_FLAT_OFF || sch->precision != AH_FP64))
         return AH_ERR_UNSUPPORTED;
@@ -466,6 +469,7 @@ static SweepArgs base_args(ah_plan* p) {
     a.flattener = p->sch.flattener;
     a.parity_override = -1;
     a.compat = p->sch.compat;
+    a.av = p->sch.av;
     return a;
 }
 
@@ -1134,6 +1138,7 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
         (sch->dims == 2 && !flux1) || (parity != 0 && parity != 1))
         return AH_ERR_INVALID;
     if (sch->magnetic_2d) return AH_ERR_UNSUPPORTED;
+    if (sch->av && (sch->dims != 1 || sch->recon != AH_PPM)) return AH_ERR_UNSUPPORTED;
     const int n0 = q->n0, n1 = sch->dims == 1 ? 1 : q->n1;
     if (n0 <= 0 || n1 <= 0 || q->dx <= 0 || q->checkpoints <= 0 || (q->boundary != AH_EDGE && q->boundary != AH_WRAP))
         return AH_ERR_INVALID;
@@ -1226,6 +1231,7 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
     a.flattener = sch->flattener;
     a.parity_override = parity;
     a.compat = sch->compat;
+    a.av = sch->av;
     a.X = grid;
     a.stage0 = 1;
     int dev = 0, cus = 256;

@@ -75,6 +75,7 @@ struct SweepArgs {
     int parity_override;  // -1: use per-problem parity
     int last;             // 2D: this flux launch finishes L
     int compat;           // 0: reference solver axis (enumeration index), 1: physical axis
+    int av;               // 1D PPM: add the artificial-viscosity flux alpha nu (q_k - q_{k-1})
     // 2D second sweep of the middle SSPRK(3,3) stage: write K2 = (3U + K1 + dt L) / 4
     // (evolvers.py:172) in place of L, in the combine kernel's operation order
     double* fuse_k2;      // null: write L
@@ -569,7 +570,31 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         const R lam = nmax(rad[i], rad[i + 1]);
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = W(e(k)).c[5];
-        const V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR, 1, a.compat != 0, SPLIT);
+        V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR, 1, a.compat != 0, SPLIT);
+        if constexpr (RECON == PPM) {
+            if (a.av) {
+                // artificial viscosity at interface k (schemes/ppm.py:138-170,
+                // McCorquodale & Colella 2011 eq. 35-38): lambda = u_k - u_{k-1}
+                // of the cell averages, nu = min(1, (dx lambda)^2 / (beta c_min^2))
+                // lambda where lambda < 0 (else 0), c_min the smaller sound speed
+                // sqrt(gamma p / rho) of the two cells, alpha = beta = 0.3; the
+                // flux gains alpha nu (q_k - q_{k-1}), the diffusive sign
+                const V8<R> am = W(e(k - 1)), ap = W(e(k));
+                const R lam_av = ap.c[1] - am.c[1];
+                if (!(lam_av >= R(0))) {
+                    const R cm = ah_sqrt(gdiv(gR * am.c[4], am.c[0])), cp = ah_sqrt(gdiv(gR * ap.c[4], ap.c[0]));
+                    const R cmin = nmin(cm, cp);
+                    const R dl = R(pd.dx) * lam_av;
+                    const R nu = nmin(R(1), gdiv(dl * dl, R(0.3) * (cmin * cmin))) * lam_av;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (V::live(c)) {
+                            const R dq = R(qd[V::slot(c)][e(k) - bq] - qd[V::slot(c)][e(k - 1) - bq]);
+                            F.c[c] = F.c[c] + R(0.3) * (nu * dq);
+                        }
+                }
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) fl[V::slot(c)][i] = F.c[c];

@@ -6,7 +6,10 @@ functions/generic.py:159-286 that the update path reads, so a parameters.yml
 written for the reference selects the same scheme here.  Keys the build adds:
 ``limiter`` (PLM slope limiter; the reference hard-codes minmod,
 schemes/plm.py:66), ``flattener`` (PPM slope flattener: off | astrea |
-colella90), ``precision_mode`` (fp64 | split) and ``max_steps``.
+colella90), ``precision_mode`` (fp64 | split), ``max_steps`` and ``av``
+(artificial viscosity on the 1D PPM fluxes; the reference computes it only
+with a flag no option sets and never applies it, so the default False is its
+behaviour).
 """
 import itertools
 from collections import namedtuple
@@ -62,7 +65,7 @@ DEFAULTS = {"config": "sod", "dimension": 1, "precision": "float64", "subgrid": 
 
 SimVariables = namedtuple("SimVariables", [
     "config", "cells", "cfl", "gamma", "dimension", "precision", "subgrid", "timestep", "solver", "run_type",
-    "checkpoints", "limiter", "flattener", "precision_mode", "max_steps", "config_category", "solver_category",
+    "checkpoints", "limiter", "flattener", "precision_mode", "max_steps", "av", "config_category", "solver_category",
     "magnetic_2d", "permutations", "ortho_axis", "boundary", "dx", "t_end",
     "start_pos", "end_pos", "shock_pos", "misc", "initial_left", "initial_right"])
 
@@ -93,6 +96,8 @@ def handle_variables(params, boundary=None, dx=None, t_end=None):
         elif k in ("gamma", "cfl"):
             if not isinstance(v, float):
                 v = 1.4 if k == "gamma" else .5
+        elif k == "av":
+            v = v if isinstance(v, bool) else False
         elif k in ACCEPTED:
             v = v.lower() if isinstance(v, str) else v
             if _category(k, v) is None:
@@ -106,6 +111,7 @@ def handle_variables(params, boundary=None, dx=None, t_end=None):
     out.setdefault("gamma", 1.4)
     out.setdefault("checkpoints", 1)
     out.setdefault("max_steps", -1)
+    out.setdefault("av", False)
     if out["gamma"] == 1:
         # generic.py:193-194: the machine epsilon of the parameters' own precision
         out["gamma"] = out["gamma"] + float(np.finfo(params.get("precision", "float64")).eps)

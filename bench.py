@@ -1,18 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the astrea finite-volume update on one MI355X.
 
-Workload (default): the 2D shear layer at 1024 x 1024, periodic, gamma 1.4,
-PPM (McCorquodale-Colella limiter) with the slope flattener, HLLC, SSPRK(3,3),
-CFL 0.4 -- one "step" is one full time step of that grid (3 RK stages x 2
-sweeps), inputs resident in HBM.  HLLC runs with the physical sweep axis
-(compat "physical"): with the reference's enumeration-index axis the scheme
-reaches negative pressure on this problem within ~20 steps at any
-resolution (the oracle reproduces that).  Two device paths run from the same initial
-state: the plain FP64 path and the split path (FP32 reconstruction, limiter
-and Riemann fluxes; FP64 state, stage combinations, pressure recovery and dt
-reduction).  ``value`` is the zone-updates/s of the faster path whose relative
-L1 error against the FP64 path after the same steps is <= 1e-5 (the FP64
-device path equals the numpy oracle to round-off, see tests/).
+Workload of ``value`` (default): the 2D shear layer at 1024 x 1024, periodic,
+gamma 1.4, PPM (McCorquodale-Colella limiter) with the slope flattener, HLLC,
+SSPRK(3,3), CFL 0.4 -- one "step" is one full time step of that grid (3 RK
+stages x 2 sweeps), inputs resident in HBM.  HLLC runs with the physical
+sweep axis (compat "physical"): with the reference's enumeration-index axis
+the scheme reaches negative pressure on this problem within ~20 steps at any
+resolution (the oracle reproduces that).  Two device paths run from the same
+initial state: the plain FP64 path and the split path (FP32 Riemann
+coefficient algebra and spectral radius, FP64 values, reconstruction, state,
+stage combinations and dt reduction).  ``value`` is the zone-updates/s of the
+faster path whose relative L1 error against the FP64 path after the same
+steps is <= 1e-5 (the FP64 device path equals the numpy oracle to round-off,
+see tests/).
+
+The same line also carries the metric's batched and served halves, bounded
+to fit a few minutes: ``ensemble`` (the 512-problem ragged 1D Riemann
+ensemble, batched in one plan against the plain one-problem-at-a-time loop,
+with the split path's L1 and the FP64 path's own sensitivity to a 1e-15
+perturbation of its initial state) and ``serve`` (a stream of 2D
+four-quadrant requests: continuous batching against the plain FIFO loop,
+saturated for throughput and Poisson-timed for p50/p99 latency), and the CPU
+baselines of every workload (``cpu_baselines``).
 
 Prints one JSON line.  Only one GPU is used: --gpus N with N > 1 exits.
 """
@@ -56,6 +66,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=3.0,
                     help="serve: arrivals per second (3.0/s loads the plain FP64 loop to ~83%%: the stream "
                          "averages 2.4e8 zone-updates per request at ~8.6e8 ZU/s)")
+    ap.add_argument("--serve-requests", type=int, default=60, help="default line: requests of the served stream")
+    ap.add_argument("--legs", default="ensemble,serve,cpu",
+                    help="default line: batched/served/CPU legs to add after the shear layer ('' for none)")
     ap.add_argument("--serve-cap", type=int, default=3 << 20,
                     help="serve: active-cell cap of the extra capped Poisson run (three 1024^2 requests)")
     return ap.parse_args()
@@ -129,6 +142,217 @@ print(json.dumps(dict(seconds=dt, zu=r.steps * {n} * {n})))
                        f"BLAS thread, CPU: {cpu}",
                 all_cores=None if r_all is None else dict(value=r_all["zu"] / r_all["seconds"], cores=nall,
                                                           seconds=r_all["seconds"]))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_child(body, timeout=600):
+    """Run oracle code in a child process pinned to one core with one BLAS
+    thread; ``body`` sets ``zu`` and ``seconds``.  Returns their dict."""
+    code = f"""
+import os, sys, time, json
+cpus = sorted(os.sched_getaffinity(0))
+os.sched_setaffinity(0, {{cpus[0]}})
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from oracle import astrea_oracle as O
+from astrea_amd import ic
+import torch
+{body}
+print(json.dumps(dict(seconds=seconds, zu=zu)))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               PYTHONWARNINGS="ignore")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=timeout)
+    if out.returncode != 0:
+        return {"error": out.stderr.strip().splitlines()[-1] if out.stderr.strip() else "failed"}
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def cpu_baselines(args):
+    """CPU baselines of the batched, served and 1D workloads: the oracle
+    (numpy restatement, the reference's LAPACK eigenvalue solve per
+    interface) on one pinned core with one BLAS thread.  The 512-cell Sod
+    tube runs in full; the ensemble and the served stream are timed on a
+    stratified sample of their problems for a bounded number of steps (the
+    oracle's rate per zone-update hardly depends on the grid size)."""
+    out = {"cores": 1, "kind": "port", "cpu": _cpu_model()}
+    sod = f"""N = 512
+x = (np.arange(N) + .5) / N
+w = np.zeros((N, 8)); w[:] = [.125, 0, 0, 0, .1, 0, 0, 0]; w[x <= .5] = [1, 0, 0, 0, 1, 0, 0, 0]
+s = O.Scheme(dimension=1, subgrid='plm', solver='llf', timestep='ssprk(3,3)', boundary='edge', gamma=1.4, cfl=.8,
+             dx=1 / N, eig='lapack')
+q = O.initial_average(w, s)
+t0 = time.perf_counter(); r = O.run(q, s, .2); seconds = time.perf_counter() - t0
+zu = r.steps * N
+"""
+    r = _cpu_child(sod)
+    if "zu" in r:
+        r["value"] = r["zu"] / r["seconds"]
+    out["sod512_full_run"] = dict(r, unit="zone-updates/s",
+                                  sample="the whole run: 512 cells, PLM-minmod/LLF/SSPRK(3,3), CFL 0.8, to t = 0.2")
+    ens = f"""specs = sorted(ic.ensemble_1d({args.nprob}, {SEED}), key=lambda sp: sp['n'])
+pick = [specs[0], specs[len(specs) // 2], specs[-1]]
+zu, seconds = 0, 0.0
+for sp in pick:
+    g = ic.initial_average(torch.from_numpy(ic.riemann_1d(sp['n'], sp['left'], sp['right'])), 1.4, 'edge', True).numpy()
+    s = O.Scheme(dimension=1, subgrid='ppm', solver='hllc', timestep='ssprk(3,3)', boundary='edge', gamma=1.4,
+                 cfl=.5, dx=1 / sp['n'], eig='lapack', compat='physical')
+    t0 = time.perf_counter()
+    try:
+        r = O.run(g, s, sp['t_end'], max_steps=20)
+        n = r.steps
+    except np.linalg.LinAlgError:
+        n = 0
+    seconds += time.perf_counter() - t0
+    zu += n * sp['n']
+"""
+    r = _cpu_child(ens)
+    if "zu" in r:
+        r["value"] = r["zu"] / r["seconds"]
+    out["ensemble_sample"] = dict(r, unit="zone-updates/s",
+                                  sample="the smallest, median and largest of the 512 ensemble problems, "
+                                         "PPM/HLLC/SSPRK(3,3), 20 steps each (or to their t_end)")
+    srv = f"""reqs = ic.quadrant_requests(64, {SEED})
+pick = []
+for n in (128, 256):
+    for sg in ('plm', 'ppm'):
+        pick.append(next(r for r in reqs if r['n'] == n and r['subgrid'] == sg))
+zu, seconds = 0, 0.0
+for sp in pick:
+    n = sp['n']
+    g = ic.initial_average(torch.from_numpy(ic.quadrants(n, sp['states'])), 1.4, 'edge', sp['subgrid'] == 'ppm').numpy()
+    s = O.Scheme(dimension=2, subgrid=sp['subgrid'], solver='hllc', timestep='ssprk(3,3)', boundary='edge', gamma=1.4,
+                 cfl=.4, dx=1 / n, eig='lapack', compat='physical')
+    t0 = time.perf_counter()
+    r = O.run(g, s, .25, max_steps=2)
+    seconds += time.perf_counter() - t0
+    zu += r.steps * n * n
+"""
+    r = _cpu_child(srv)
+    if "zu" in r:
+        r["value"] = r["zu"] / r["seconds"]
+    out["serve_sample"] = dict(r, unit="zone-updates/s",
+                               sample="four requests of the served stream (128^2 and 256^2, PLM and PPM; HLLC, "
+                                      "SSPRK(3,3)), 2 steps each")
+    return out
+
+
+def ensemble_leg(args):
+    """The ragged 1D Riemann ensemble (C4) for the default line: batched
+    FP64 and split against the plain loop on a stratified sample, the split
+    path's L1 per completed problem, and the FP64 path's own L1 when its
+    initial state is perturbed by a relative +-1e-15 (what any reduced
+    precision can at best reach on these problems)."""
+    import numpy as np
+    import torch
+    from astrea_amd import workloads as W
+    grids, probs = W.ensemble_inputs(args.nprob, SEED)
+    blocks, tune = W.autotune_blocks(grids, probs, "fp64")
+    res = {p: W.run_ensemble_batched(grids, probs, precision=p, blocks=blocks) for p in ("fp64", "split")}
+    _progress(f"[ensemble] batched fp64 {res['fp64']['zu_per_s']:.3e} ZU/s, split {res['split']['zu_per_s']:.3e}")
+    host, _ = W.ensemble_inputs(args.nprob, SEED, host_grids=True)
+    gen = torch.Generator(device=host[0].device).manual_seed(SEED)
+    pert = []
+    for g in host:
+        sign = 2 * torch.randint(0, 2, g.shape, generator=gen, device=g.device, dtype=torch.float64) - 1
+        pert.append(g * (1 + 1e-15 * sign))
+    assert all(not torch.equal(a, b) for a, b in zip(pert, host))
+    res_p = W.run_ensemble_batched(pert, probs, precision="fp64", blocks=blocks)
+    del host, pert
+    ref = res["fp64"]
+    split_err, sens, mism = {}, {}, 0
+    for i in range(len(probs)):
+        f_ref, f_s, f_p = (bool(r["status"][i]["failed"]) for r in (ref, res["split"], res_p))
+        o, n = ref["offsets"][i], probs[i]["n0"]
+        if f_ref != f_s:
+            mism += 1
+        elif not f_ref:
+            split_err[i] = W.rel_l1(res["split"]["state"][:, o:o + n], ref["state"][:, o:o + n])
+        if not f_ref and not f_p:
+            sens[i] = W.rel_l1(res_p["state"][:, o:o + n], ref["state"][:, o:o + n])
+    rng = np.random.default_rng(SEED)
+    order = sorted(range(len(probs)), key=lambda i: probs[i]["n0"])
+    strata = np.array_split(np.array(order), args.plain_sample)
+    sample = sorted(int(rng.choice(st)) for st in strata if len(st))
+    plain = W.run_ensemble_plain(grids, probs, sample=sample)
+    _progress(f"[ensemble] plain fp64 sample {plain['zu_per_s']:.3e} ZU/s")
+    stat = lambda v: None if not v else {"median": float(np.median(v)), "p90": float(np.percentile(v, 90)),
+                                         "max": float(np.max(v)), "n": len(v)}
+    worst = sorted(split_err, key=split_err.get, reverse=True)[:5]
+    split_ok = bool(split_err) and max(split_err.values()) <= 1e-5 and mism == 0
+    best = "split" if split_ok and res["split"]["zu_per_s"] > ref["zu_per_s"] else "fp64"
+    return {
+        "workload": f"{args.nprob} ragged 1D Riemann problems, N log-uniform [256,8192], PPM/HLLC (physical axis)/"
+                    "SSPRK(3,3), CFL 0.5, each to its own t_end, one batched plan (device-built initial states, "
+                    "graph chunks of 32 steps, compaction every 4th)",
+        "path": best, "zu_per_s": res[best]["zu_per_s"],
+        "plain_fp64_zu_per_s": plain["zu_per_s"], "speedup_vs_plain": res[best]["zu_per_s"] / plain["zu_per_s"],
+        "plain_sample": {"problems": len(sample), "seconds": plain["seconds"],
+                         "how": "one problem per size stratum, sizes sorted into equal strata"},
+        "batched": {p: {"zu_per_s": r["zu_per_s"], "seconds": r["seconds"], "graph_chunks": r["chunks"],
+                        "failed": sum(s["failed"] for s in r["status"])} for p, r in res.items()},
+        "split_l1_vs_fp64": dict(stat(list(split_err.values())) or {}, failure_mismatch=mism),
+        "fp64_l1_under_1e-15_perturbation": stat(list(sens.values())),
+        "worst_split_problems": [{"problem": i, "n": probs[i]["n0"], "split_l1": split_err[i],
+                                  "fp64_perturbed_l1": sens.get(i)} for i in worst],
+        "autotune_blocks": {"picked": blocks, "seconds": {str(k): v for k, v in tune.items()}},
+    }
+
+
+def serve_leg(args):
+    """A bounded served stream (C5) for the default line: ``--serve-requests``
+    four-quadrant requests, all queued at t = 0 for throughput (served
+    against the plain FIFO loop on the same requests, results compared), then
+    Poisson arrivals at ``--rate`` for p50/p99 latency."""
+    from astrea_amd import serve
+    from astrea_amd.workloads import rel_l1
+    nreq = args.serve_requests
+
+    def requests(realtime):
+        reqs = serve.make_requests(nreq, SEED, args.rate)
+        if not realtime:
+            for r in reqs:
+                r.arrival = 0.0
+        return reqs
+
+    plain = serve.run_fifo(requests(False), realtime=False, keep_results=True)
+    ref = {r.rid: r for r in plain.pop("done")}
+    srv = serve.Server(keep_results=True).run(requests(False), realtime=False)
+    errs, mism = [], 0
+    for x in srv.pop("done"):
+        y = ref[x.rid]
+        if bool(x.status["failed"]) != bool(y.status["failed"]) or x.status["steps"] != y.status["steps"]:
+            mism += 1
+        elif not y.status["failed"]:
+            errs.append(rel_l1(x.result, y.result))
+    _progress(f"[serve] saturated: plain {plain['zu_per_s']:.3e}, served {srv['zu_per_s']:.3e} ZU/s")
+    plain_p = serve.run_fifo(requests(True), realtime=True)
+    plain_p.pop("done")
+    srv_p = serve.Server().run(requests(True), realtime=True)
+    srv_p.pop("done")
+    _progress(f"[serve] poisson: plain p50 {plain_p['p50']:.2f} s p99 {plain_p['p99']:.2f} s, "
+              f"served p50 {srv_p['p50']:.2f} s p99 {srv_p['p99']:.2f} s")
+    return {
+        "workload": f"{nreq} four-quadrant 2D requests, n uniform in {{128,256,512,1024}}^2, PLM or PPM, HLLC "
+                    "(physical axis), SSPRK(3,3), CFL 0.4, t_end 0.25, edge; continuous batching, FP64",
+        "saturated": {"plain_fp64_zu_per_s": plain["zu_per_s"], "served_fp64_zu_per_s": srv["zu_per_s"],
+                      "speedup_vs_plain": srv["zu_per_s"] / plain["zu_per_s"], "rebuilds": srv["rebuilds"],
+                      "l1_vs_plain_max": max(errs) if errs else None, "compared": len(errs),
+                      "step_or_failure_mismatch": mism, "failed": srv["failed"]},
+        "poisson": {"rate_per_s": args.rate, "plain_p50_s": plain_p["p50"], "plain_p99_s": plain_p["p99"],
+                    "served_p50_s": srv_p["p50"], "served_p99_s": srv_p["p99"],
+                    "plain_zu_per_s": plain_p["zu_per_s"], "served_zu_per_s": srv_p["zu_per_s"]},
+    }
 
 
 def bench_ensemble(args):
@@ -443,11 +667,23 @@ def main():
     b_alg_zu = 8 * 4 * 8
     per_launch_bytes = b_alg_zu * n * n / 6
     achieved = per_launch_bytes / (dom_ms / 1e3) / 1e9
-    traffic = None
+    # HBM bytes and VALU instructions per launch of that kernel come from the
+    # rocprofv3 counter runs summarised in profiles/ (1024^2 only); the VALU
+    # fraction is the issue time of those instructions (4 cycles per wave64
+    # instruction on each of the 1024 SIMDs at 2.4 GHz) over the live launch time
+    traffic, valu = None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_shear1024.json")
-    if os.path.exists(tpath):
+    if n == 1024 and os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        key = "" if best == "fp64" else "_split"
+        traffic = tj.get("bytes_per_launch" + key)
+        insts = tj.get("valu_wave_insts_per_launch" + key)
+        if insts:
+            issue_s = insts * 4 / (1024 * 2.4e9)
+            valu = {"wave_insts_per_launch": insts, "issue_ms": issue_s * 1e3, "frac": issue_s / (dom_ms / 1e3),
+                    "source": tj.get("source")}
+    legs = [x for x in args.legs.split(",") if x]
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
     line = {
         "metric": METRIC,
@@ -468,7 +704,10 @@ def main():
                    "launch": f"hipGraph replay of {chunk}-step chunks (per-kernel times from an eager pass)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": dom,
-                     "kernel_avg_ms": dom_ms, "bytes_per_launch": per_launch_bytes},
+                     "kernel_avg_ms": dom_ms, "bytes_per_launch": per_launch_bytes,
+                     "traffic_source": None if traffic is None else
+                     f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {dom} at 1024^2, calibrated ({tj.get('source')})",
+                     "valu": valu},
         "cpu_baseline": cpu,
         "paths": {p: {"zu_per_s": zu / (v["ms"] / 1e3), "ms_per_step": v["ms"] / args.steps,
                       "eager_ms_per_step": v["eager_ms_per_step"], "l1_vs_fp64": l1.get(p, 0.0),
@@ -477,9 +716,19 @@ def main():
                   for p, v in results.items()},
         "step_hbm_achieved_GBps": b_alg_zu * value / 1e9,
     }
-    print(json.dumps(line))
     for v in results.values():
         v["plan"].close()
+    del results
+    torch.cuda.empty_cache()
+    if "ensemble" in legs:
+        line["ensemble"] = ensemble_leg(args)
+        torch.cuda.empty_cache()
+    if "serve" in legs:
+        line["serve"] = serve_leg(args)
+        torch.cuda.empty_cache()
+    if "cpu" in legs and not args.no_cpu:
+        line["cpu_baselines"] = cpu_baselines(args)
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -93,7 +93,10 @@ typedef struct ah_scheme {
     int tile_s;      /* sweep tile length in cells (0: default / autotuned) */
     int tile_t;      /* transverse tile width in lines, 2D only (0: default) */
     int compat;      /* ah_compat */
-    int reserved[4];
+    int av;          /* 1: artificial viscosity on the fluxes (1D PPM only; schemes/ppm.py:138-170,
+                        whose output the reference computes but never applies: 0 is the
+                        reference-equivalent setting) */
+    int reserved[3];
 } ah_scheme;
 
 typedef struct ah_problem {

@@ -57,7 +57,9 @@ PARITY = {
                             "space_2d x sweep, consistency at axis 1, upwind supersonic limits, "
                             "transposition equivariance, convergence to the exact Riemann solution",
     "hlld compat=physical": "golden space_2d p0 (the reference's call at the physical axis)",
-    "artificial viscosity": "parity unpinned (reference output never read, schemes/ppm.py:95)",
+    "artificial viscosity (mu)": "golden pins.npz av_strong_* (reference apply_artificial_viscosity, the "
+                                 "N = 8 case it runs on, saturated coefficient) + known answer of eq. 36",
+    "artificial viscosity (flux hook)": "build option, off by default (the reference never applies mu)",
 }
 
 
@@ -81,6 +83,7 @@ class Scheme:
     parity: int = 0               # step parity: sweep order flips every step
     compat: str = "astrea"        # astrea: reference quirks | physical: solver axis = sweep axis
     category: str = ""            # solver_category when it differs from the name's (generic.py:241,266-269)
+    av: bool = False              # artificial viscosity on the fluxes (1D PPM; the reference never applies it)
 
     @property
     def high_order(self):
@@ -422,6 +425,44 @@ def flattener_eta(wS, axis, bc, variant, delta=.33, z0=.75, z1=.85):
     eta[sg < 0] = np.minimum(chi, cp[2:])[sg < 0]
     eta[sg > 0] = np.minimum(chi, cp[:-2])[sg > 0]
     return eta
+
+
+def artificial_viscosity(wS, axis, s, alpha=.3, beta=.3):
+    """schemes/ppm.py:138-170 (McCorquodale & Colella 2011 eq. 35-38) per
+    face i+1/2, i = 0..N-1, in 1D: lambda = u_{i+1} - u_i (padded), c_min the
+    smaller of the two cells' sqrt(fv.divide(gamma p, rho)), nu = min(1,
+    (dx lambda)^2 / (beta c_min^2)) lambda, zero where lambda >= 0, and
+    mu = alpha (nu (w_{i+1} - w_i)).  This is the reference's output where
+    its broadcast lets it run (N = 8, where nu[i, j] = coefficient_j lambda_i
+    equals the per-face value whenever the coefficient is the same on every
+    face, e.g. saturated at 1); its 2D branch adds arrays of different shapes
+    (ppm.py:149-154) and never runs.  Nothing in the reference reads mu."""
+    if wS.ndim != 2:
+        raise NotImplementedError("the reference's 2D artificial viscosity cannot run (ppm.py:149-154)")
+    w = pad(wS, s.boundary)
+    lam = w[2:, axis + 1] - w[1:-1, axis + 1]
+    cs = np.sqrt(gdiv(s.gamma * w[..., 4], w[..., 0]))
+    cmin = np.minimum(cs[1:-1], cs[2:])
+    nu = np.minimum(1, gdiv((s.dx * lam) ** 2, beta * cmin ** 2)) * lam
+    nu[lam >= 0] = 0
+    return alpha * (nu[:, None] * np.diff(w[1:], axis=0))
+
+
+def av_flux(q, wS, axis, s, alpha=.3, beta=.3):
+    """The artificial-viscosity flux the device adds when the option is on,
+    at every interface k = 0..N (cells k-1 and k, padded): alpha (nu (q_k -
+    q_{k-1})) with nu of ``artificial_viscosity`` and the conservative cell
+    averages q (M&C 2011 eq. 38 applies nu to the conserved variables; the
+    reference's mu takes differences of the primitive ones).  nu <= 0, so
+    adding this term to the flux is diffusive."""
+    w = pad(wS, s.boundary)
+    qq = pad(q, s.boundary)
+    lam = w[1:, axis + 1] - w[:-1, axis + 1]
+    cs = np.sqrt(gdiv(s.gamma * w[..., 4], w[..., 0]))
+    cmin = np.minimum(cs[:-1], cs[1:])
+    nu = np.minimum(1, gdiv((s.dx * lam) ** 2, beta * cmin ** 2)) * lam
+    nu[lam >= 0] = 0
+    return alpha * (nu[:, None] * (qq[1:] - qq[:-1]))
 
 
 # ----------------------------------------------------------------------------
@@ -1048,6 +1089,8 @@ def evolve_space(grid, s):
             F = np.copy(Fc) - C24 * lap(np.copy(Favg), s.boundary, 1)
         else:
             F = Favg
+            if s.av and s.subgrid in ("ppm", "parabolic", "p"):
+                F = F + av_flux(grid.transpose(perm(ax, 1)), d["wS"], ax, s)
         fluxes[ax] = {"flux": F, "eigmax": eigmax}
     if s.magnetic_2d:
         # evolvers.py:26-32: each sweep keeps its own wF; the corner EMF is

@@ -346,3 +346,45 @@ def test_magnetic_config_with_hllc_runs_hllc():
     assert np.max(rel_l1(out, ref.grid)) <= 1e-12
     llf = O.run(q0, replace(s, category=""), sv.t_end, max_steps=12)
     assert not np.array_equal(ref.grid, llf.grid)
+
+
+# ---- artificial viscosity (schemes/ppm.py:138-170), opt-in ---------------------
+@pytest.mark.parametrize("bc,solver", [("wrap", "hllc"), ("edge", "llf")])
+def test_artificial_viscosity_trajectory(bc, solver):
+    """1D PPM with the artificial-viscosity flux switched on: 30 steps of a
+    compressive profile equal the oracle's (whose viscosity coefficient is
+    pinned to the reference's apply_artificial_viscosity), the interface
+    fluxes of ah_evolve_space too, and the option changes the result."""
+    from astrea_amd.plan import Plan
+    n = 96
+    x = (np.arange(n) + .5) / n
+    w = np.zeros((n, 8))
+    w[:, 0] = 1 + .5 * np.exp(-((x - .5) / .1) ** 2)
+    w[:, 1] = -np.sin(2 * np.pi * x)
+    w[:, 4] = 1. + .2 * (x < .3)
+    s = O.Scheme(dimension=1, subgrid="ppm", solver=solver, boundary=bc, gamma=1.4, cfl=.4, dx=1 / n, av=True)
+    q0 = O.initial_average(w, s)
+    _say(f"[gpu] artificial viscosity {bc} {solver}")
+    prob = dict(n0=n, boundary=bc, dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9, max_steps=30)
+    plan = Plan(1, "ppm", solver, "ssprk(3,3)", [prob], varmask=0x13, av=True)
+    plan.load([q0])
+    fl = plan.evolve_space(0)
+    d = O.evolve_space(q0, s)
+    assert _scaled_err(fl[0]["flux"], d[0]["flux"]) <= 1e-14
+    plan.reset()
+    st = plan.run(32)[0]
+    out = plan.unpack(0)
+    plan.close()
+    ref = O.run(q0, s, 1e9, max_steps=30)
+    assert st["steps"] == ref.steps == 30 and not st["failed"]
+    assert np.max(rel_l1(out, ref.grid)) <= 1e-12
+    assert not np.array_equal(ref.grid, O.run(q0, replace(s, av=False), 1e9, max_steps=30).grid)
+
+
+def test_artificial_viscosity_refused_outside_1d_ppm():
+    from astrea_amd import _lib as L
+    from astrea_amd.plan import Plan
+    for dims, sg in ((1, "plm"), (2, "ppm")):
+        prob = dict(n0=16, n1=16, boundary="wrap", dx=1 / 16, gamma=1.4, cfl=.4, t_end=1.)
+        with pytest.raises(L.AstreaHipError, match="not implemented"):
+            Plan(dims, sg, "hllc", "ssprk(3,3)", [prob], varmask=0x13 if dims == 1 else 0x17, av=True)

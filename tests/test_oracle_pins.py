@@ -274,3 +274,60 @@ def test_physical_hlld_is_the_reference_call_at_the_physical_axis():
         for ax in (0, 1):
             assert np.array_equal(fl[ax]["flux"], g[f"weno5_hlld_wrap_p0_flux_ax{ax}"]), (parity, ax)
     assert not np.array_equal(g["weno5_hlld_wrap_p1_flux_ax0"], g["weno5_hlld_wrap_p0_flux_ax0"])
+
+
+# ---- artificial viscosity (schemes/ppm.py:138-170) ---------------------------
+@pytest.mark.parametrize("bc", ["edge", "wrap"])
+def test_artificial_viscosity_equals_reference_where_it_runs(bc):
+    """The reference's apply_artificial_viscosity runs only with 8 cells
+    (its broadcast), where its nu[i, j] = coefficient_j * lambda_i equals the
+    per-face value when the coefficient min(1, (dx lambda)^2/(beta c^2)) is
+    the same on every face that carries one: strong compression saturates it
+    at 1.  The restatement equals the reference there bit for bit."""
+    wS = PINS[f"av_strong_{bc}_wS"]
+    s = O.Scheme(dimension=1, subgrid="ppm", boundary=bc, gamma=1.4, dx=1 / 8)
+    mu = O.artificial_viscosity(wS, 0, s)
+    assert np.array_equal(mu, PINS[f"av_strong_{bc}_mu"])
+    assert np.any(mu != 0)
+
+
+def test_artificial_viscosity_known_answer():
+    """Eq. 36 by hand: u falling by 1 per cell, rho = 1.4, p = 1 (c = 1),
+    dx = .1, beta = .3: nu = min(1, .01/.3) (-1) = -1/30 on the interior
+    faces, so mu = alpha nu dw = -.01 dw; the last face (edge ghost, lambda
+    = 0) and expanding faces carry none."""
+    w = np.zeros((5, 8))
+    w[:, 0] = 1.4
+    w[:, 4] = 1.
+    w[:, 1] = [0., -1., -2., -3., -2.]
+    s = O.Scheme(dimension=1, subgrid="ppm", boundary="edge", gamma=1.4, dx=.1)
+    mu = O.artificial_viscosity(w, 0, s)
+    dw = np.diff(np.vstack([w, w[-1:]]), axis=0)
+    np.testing.assert_allclose(mu[:3], -.01 * dw[:3], rtol=1e-12)
+    assert np.all(mu[3:] == 0)
+    # the flux hook uses the same nu on the conserved differences, per interface
+    q = O.prim_to_cons(w, 1.4)
+    F = O.av_flux(q, w, 0, s)
+    qq = O.pad(q, "edge")
+    np.testing.assert_allclose(F[1:4], -.01 * (qq[2:5] - qq[1:4]), rtol=1e-12)
+    assert np.all(F[[0, 4, 5]] == 0)
+
+
+def test_artificial_viscosity_hook_is_diffusive():
+    """With the hook on, a compressive smooth profile loses total variation
+    faster than without it, and the hook changes nothing in expansion."""
+    N = 64
+    x = (np.arange(N) + .5) / N
+    w = np.zeros((N, 8))
+    w[:, 0] = 1 + .5 * np.exp(-((x - .5) / .1) ** 2)
+    w[:, 1] = -np.sin(2 * np.pi * x)            # converging at x = .5
+    w[:, 4] = 1.
+    kw = dict(dimension=1, subgrid="ppm", solver="hllc", boundary="wrap", gamma=1.4, cfl=.4, dx=1 / N)
+    s0, s1 = O.Scheme(**kw), O.Scheme(**kw, av=True)
+    q = O.initial_average(w, s0)
+    tv = lambda g: np.abs(np.diff(O.to_prim(g, s0)[:, 0])).sum()
+    a, b = O.run(q, s0, 1e9, max_steps=40), O.run(q, s1, 1e9, max_steps=40)
+    assert tv(b.grid) < tv(a.grid)
+    w[:, 1] = np.sin(2 * np.pi * x) * 0 + .3    # uniform flow: no compression anywhere
+    q = O.initial_average(w, s0)
+    assert np.array_equal(O.run(q, s0, 1e9, max_steps=5).grid, O.run(q, s1, 1e9, max_steps=5).grid)
