@@ -294,8 +294,12 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
                         for (int s0 = 0; s0 < ns; s0 += p->ks.tts) p->tl[6 + ax].full.push_back({i, s0, t0, 0});
             }
         }
-        for (long long c0 = 0; c0 < cells; c0 += 1024)
-            p->tl[5].full.push_back({i, (int)c0, (int)std::min<long long>(1024, cells - c0), 0});
+        // combination tiles: 256 cells (one per thread) in 1D, so the few large
+        // problems left at the end of an ensemble still spread over many CUs;
+        // 1024 in 2D, where four cells per thread ran faster on 1024^2 grids
+        const int ct = sch->dims == 1 ? 256 : 1024;
+        for (long long c0 = 0; c0 < cells; c0 += ct)
+            p->tl[5].full.push_back({i, (int)c0, (int)std::min<long long>(ct, cells - c0), 0});
     }
     p->total = total;
     if (total >= (1ll << 31)) {
@@ -477,9 +481,11 @@ static int blocks_for(const ah_plan* p, const TileList& L) { return std::min<int
 
 // L of the stage input X for every active problem (evolve_space + compute_L)
 static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, int parity_override,
-                         double* flux0, double* flux1, hipStream_t s, bool fuse33 = false) {
+                         double* flux0, double* flux1, hipStream_t s, bool fuse33 = false,
+                         const double* xl = nullptr) {
     SweepArgs a = base_args(p);
     a.X = X;
+    a.xl = xl;
     a.stage0 = stage0;
     a.parity_override = parity_override;
     if (fuse33) {
@@ -586,9 +592,19 @@ static void launch_step(ah_plan* p, hipStream_t s) {
         const Stage& S = p->prog[st];
         // the middle SSPRK(3,3) stage combines inside the kernel that finishes L
         // (the 1D sweep, the second 2D flux sweep): dt is known after stage 0,
-        // and a failed problem's K2 is never read
+        // and a failed problem's K2 is never read.  Its input K1 = U + dt L0 is
+        // not stored either: the kernels of that stage evaluate it from U and
+        // the stage-0 L where they read it (same operations, same bits), so
+        // the first stage needs no combination launch
         const bool fuse33 = S.op == OP_33_1 && !mhd;
-        launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s, fuse33);
+        // (1D only: in 2D the two reconstruction launches of that stage would
+        // each read U and L0 in place of K1, which measured slower than the
+        // combination launch it saves -- 0.926 -> 0.946 ms per shear-layer step)
+        const bool k1_live = (S.op == OP_33_0 || S.op == OP_33_1) && !mhd && p->sch.dims == 1;
+        if (fuse33 && k1_live)
+            launch_space(p, p->regs[0], p->Lbuf, 0, -1, nullptr, nullptr, s, true, p->Lbuf);
+        else
+            launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s, fuse33);
         MhdArgs m;
         if (mhd) {
             // corner EMF from both sweeps' face values; it replaces the B rows of L
@@ -606,7 +622,7 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
         if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[7]), s);
-        if (!fuse33) launch_combine(p, S.op, s);
+        if (!fuse33 && !(k1_live && S.op == OP_33_0)) launch_combine(p, S.op, s);
         if (mhd) {
             double* K = p->regs[target_reg(S.op)];
             m.K = K;

@@ -55,6 +55,8 @@ __device__ inline double step_dt(const ProbDesc& d, const ProbState& s, int dims
 
 struct SweepArgs {
     const double* X;      // stage input state, packed SoA (plane stride `plane`)
+    const double* xl;     // non-null: the stage input is X + dt xl, the first SSPRK(3,3)
+                          // register K1 = U + dt L0 evaluated where it is read
     double* out;          // 1D: L;  2D flux: D (first sweep) or L (second)
     const double* Dprev;  // 2D second sweep: D of the first sweep
     void* wL;             // 2D interface states, sweep-frame packed SoA, type R
@@ -82,6 +84,14 @@ struct SweepArgs {
     const double* fuse_u;
     const double* fuse_k1;
 };
+
+// Stage input at packed index j: X[j], or K1 = U + dt L0 (evolvers.py:169)
+// in the stage combination's operation order when the register is not stored
+__device__ inline double stage_x(const SweepArgs& a, long long j, double xdt) {
+#pragma clang fp contract(off)
+    if (a.xl) return a.X[j] + xdt * a.xl[j];
+    return a.X[j];
+}
 
 __device__ inline int clampi(int p, int lo, int hi) { return p < lo ? lo : (p > hi ? hi : p); }
 __device__ inline int bcmap(int p, int n, int bc) {
@@ -385,6 +395,8 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     const int sax = 0;
     auto e = [&](int p) { return bc == EDGE ? clampi(p, 0, N - 1) : p; };
     const int bq = s0 - HW_L - 1, bw = s0 - HW_L, bcl = s0 - 1;
+    int xhit_;
+    const double xdt = a.xl ? step_dt(pd, *ps, 1, &xhit_) : 0.0;
 
     // A: load the stage input (conservative) with boundary mapping
     for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
@@ -392,7 +404,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         V8<double> q = zero8<double, MASK>();
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-            if (V::live(c)) q.c[c] = a.X[V::slot(c) * P + off + g];
+            if (V::live(c)) q.c[c] = stage_x(a, V::slot(c) * P + off + g, xdt);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) qd[V::slot(c)][i] = q.c[c];
@@ -619,7 +631,8 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             const long long j = v * P + off + cell;
             if (a.fuse_k2) {
 #pragma clang fp contract(off)
-                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + fdt * L);
+                const double k1 = a.xl ? stage_x(a, j, fdt) : a.fuse_k1[j];
+                a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + k1 + fdt * L);
             } else {
                 a.out[j] = L;
             }
@@ -694,6 +707,8 @@ void recon2d_kernel(SweepArgs a) {
     auto es = [&](int p) { return bc == EDGE ? clampi(p, 0, ns - 1) : p; };
     auto et = [&](int p) { return bc == EDGE ? clampi(p, 0, nt - 1) : p; };
     const int bqs = s0 - HS - 1, bqt = t0 - HT, bws = s0 - HS;
+    int xhit_;
+    const double xdt = a.xl ? step_dt(pd, *ps, 2, &xhit_) : 0.0;
 
     // A: load, memory-contiguous index fastest
     for (int i = threadIdx.x; i < NQT * NQS; i += blockDim.x) {
@@ -705,7 +720,7 @@ void recon2d_kernel(SweepArgs a) {
         V8<double> q = zero8<double, MASK>();
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-            if (V::live(c)) q.c[c] = a.X[V::slot(c) * P + g];
+            if (V::live(c)) q.c[c] = stage_x(a, V::slot(c) * P + g, xdt);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) qd[V::slot(c)][it][is] = q.c[c];
@@ -987,7 +1002,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     if (a.stage0) block_max_atomic(lmax, &ps->eig[AX], red);
     __syncthreads();
     // C: q+-, f+- and the face-average flux on lines [t0-1, t0+TT+1)
-    const double* Xg = a.X;
+    int xhit_;
+    const double xdt = (CONV == 0 && a.xl) ? step_dt(pd, *ps, 2, &xhit_) : 0.0;
     for (int i = threadIdx.x; i < NTA * NKF; i += blockDim.x) {
         const int ia = i / NKF, ik = i % NKF;
         const int t = bta + ia, te = et(t), it = te - bt;
@@ -1002,8 +1018,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 if (V::live(c)) {
-                    qm.c[c] = Xg[V::slot(c) * P + om];
-                    qp.c[c] = Xg[V::slot(c) * P + op];
+                    qm.c[c] = stage_x(a, V::slot(c) * P + om, xdt);
+                    qp.c[c] = stage_x(a, V::slot(c) * P + op, xdt);
                 }
             f.qm = cast8<R, double, MASK>(qm);
             f.qp = cast8<R, double, MASK>(qp);
@@ -1129,7 +1145,8 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
                 if (a.fuse_k2) {
 #pragma clang fp contract(off)
                     const long long j = v * P + g;
-                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + a.fuse_k1[j] + fdt * L);
+                    const double k1 = a.xl ? stage_x(a, j, fdt) : a.fuse_k1[j];
+                    a.fuse_k2[j] = .25 * (3 * a.fuse_u[j] + k1 + fdt * L);
                 } else {
                     a.out[v * P + g] = L;
                 }

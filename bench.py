@@ -69,6 +69,12 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=60, help="default line: requests of the served stream")
     ap.add_argument("--legs", default="ensemble,serve,cpu",
                     help="default line: batched/served/CPU legs to add after the shear layer ('' for none)")
+    ap.add_argument("--ens-paths", default="fp64,split,literal",
+                    help="ensemble: batched runs (fp64, split, literal = the reference's HLLC axis and default)")
+    ap.add_argument("--serve-runs", default="saturated,poisson",
+                    help="serve: modes to run (saturated, poisson); with --serve-only a single served or plain run")
+    ap.add_argument("--serve-only", default="", choices=["", "served_fp64", "plain_fp64"],
+                    help="serve: run only this path (long Poisson traces, one per call)")
     ap.add_argument("--serve-cap", type=int, default=3 << 20,
                     help="serve: active-cell cap of the extra capped Poisson run (three 1024^2 requests)")
     return ap.parse_args()
@@ -365,13 +371,19 @@ def bench_ensemble(args):
     grids, probs = W.ensemble_inputs(args.nprob, SEED)
     blocks, tune = W.autotune_blocks(grids, probs, "fp64")
     res = {}
+    paths = args.ens_paths.split(",")
     for prec in ("fp64", "split"):
-        res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, blocks=blocks)
-        _progress(f"[ensemble] batched {prec}: {res[prec]['zu_per_s']:.3e} ZU/s")
-    literal = W.run_ensemble_batched(grids, probs, precision="fp64", blocks=blocks, compat="astrea")
+        if prec in paths or prec == "fp64":
+            res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, blocks=blocks)
+            _progress(f"[ensemble] batched {prec}: {res[prec]['zu_per_s']:.3e} ZU/s")
+    literal = None
+    if "literal" in paths:
+        literal = W.run_ensemble_batched(grids, probs, precision="fp64", blocks=blocks, compat="astrea")
     ref = res["fp64"]
     errs, mism = [], 0
     for i in range(len(probs)):
+        if "split" not in res:
+            break
         if bool(ref["status"][i]["failed"]) != bool(res["split"]["status"][i]["failed"]):
             mism += 1
             continue
@@ -397,8 +409,8 @@ def bench_ensemble(args):
         "batched": {p: {"zu_per_s": r["zu_per_s"], "seconds": r["seconds"], "graph_chunks": r["chunks"],
                         "failed": sum(s["failed"] for s in r["status"])} for p, r in res.items()},
         "plain_fp64_sample": {"zu_per_s": plain["zu_per_s"], "problems": len(sample), "seconds": plain["seconds"]},
-        "reference_literal_hllc_fp64": {"zu_per_s": literal["zu_per_s"],
-                                        "failed": sum(s["failed"] for s in literal["status"])},
+        "reference_literal_hllc_fp64": None if literal is None else
+        {"zu_per_s": literal["zu_per_s"], "failed": sum(s["failed"] for s in literal["status"])},
         "speedup_vs_plain": value / plain["zu_per_s"],
         "split_l1_vs_fp64": {"max": max(errs) if errs else None, "median": float(np.median(errs)) if errs else None,
                              "p90": float(np.percentile(errs, 90)) if errs else None, "compared": len(errs),
@@ -427,8 +439,29 @@ def bench_serve(args):
                 r.arrival = 0.0
         return reqs
 
+    if args.serve_only:
+        # one long Poisson trace of one path (the plain loop and the server
+        # each take the whole arrival window, so they run in separate calls)
+        reqs = requests(True)
+        if args.serve_only == "plain_fp64":
+            r = serve.run_fifo(reqs, realtime=True)
+        else:
+            r = serve.Server(precision="fp64").run(reqs, realtime=True)
+        r.pop("done")
+        return {"metric": METRIC, "value": r["zu_per_s"], "unit": "zone-updates/s", "n_gpus": 1, "steps": None,
+                "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64", "data": "synthetic",
+                "config": {"workload": f"{nreq} four-quadrant 2D requests, Poisson {args.rate}/s over "
+                                       f"{args.duration:.0f} s, n in {{128,256,512,1024}}^2, PLM or PPM, HLLC "
+                                       "(physical axis), SSPRK(3,3), CFL 0.4, t_end 0.25, edge",
+                           "path": args.serve_only, "seed": SEED},
+                "p50_s": r["p50"], "p99_s": r["p99"], "requests": r["requests"], "failed": r["failed"],
+                "makespan_s": r["seconds"], "rebuilds": r["rebuilds"]}
     modes = {}
+    runs_wanted = args.serve_runs.split(",")
     for mode, realtime in (("saturated", False), ("poisson", True)):
+        if mode not in runs_wanted:
+            continue
         plain = serve.run_fifo(requests(realtime), realtime=realtime, keep_results=True)
         ref = {r.rid: r for r in plain.pop("done")}
         out = {"plain_fp64": plain}
@@ -460,8 +493,8 @@ def bench_serve(args):
           and sat[p]["l1_vs_plain"]["failure_mismatch"] == 0]
     best = max(ok, key=lambda p: sat[p]["zu_per_s"]) if ok else "served_fp64"
     value = sat[best]["zu_per_s"]
-    poi = modes["poisson"]
-    return {
+    poi = modes.get("poisson")
+    line = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": None,
         "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if best == "served_fp64" else "f32+f64", "data": "synthetic",
@@ -470,11 +503,14 @@ def bench_serve(args):
                                "axis), SSPRK(3,3), CFL 0.4, t_end 0.25, edge", "path": best, "seed": SEED},
         "saturated": sat, "poisson": poi,
         "speedup_vs_plain": value / sat["plain_fp64"]["zu_per_s"],
-        "p50_s": poi[best]["p50"], "p99_s": poi[best]["p99"],
-        "plain_p50_s": poi["plain_fp64"]["p50"], "plain_p99_s": poi["plain_fp64"]["p99"],
-        "capped": {"max_active_cells": args.serve_cap, "p50_s": poi["served_fp64_capped"]["p50"],
-                   "p99_s": poi["served_fp64_capped"]["p99"], "zu_per_s": poi["served_fp64_capped"]["zu_per_s"]},
     }
+    if poi is not None:
+        line.update({"p50_s": poi[best]["p50"], "p99_s": poi[best]["p99"],
+                     "plain_p50_s": poi["plain_fp64"]["p50"], "plain_p99_s": poi["plain_fp64"]["p99"],
+                     "capped": {"max_active_cells": args.serve_cap, "p50_s": poi["served_fp64_capped"]["p50"],
+                                "p99_s": poi["served_fp64_capped"]["p99"],
+                                "zu_per_s": poi["served_fp64_capped"]["zu_per_s"]}})
+    return line
 
 
 def bench_mhd(args):
