@@ -38,6 +38,10 @@ PARITY = {
     "roe_average": "closed form + golden via steps",
     "limiters.slope (minmod..superbee)": "textbook phi(r) closed forms",
     "extrapolant_limiter mc": "golden limiter_mc.npz",
+    "interface_limiter": "golden pins.npz iflim_* (reference limiters.interface_limiter: data with and without "
+                         "local extrema)",
+    "ppm authors c / ph": "golden pins.npz ppm_c_*, ppm_ph_* (reference ppm.run with author 'c'/'ph', "
+                          "hydro and MHD, both boundaries)",
     "Osher-Solomon (DOTS) flux + right eigenvectors": "golden schemes_1d.npz *_flux_os (bitwise, 24 cases)",
     "pcm/plm/ppm/weno interface states": "golden schemes_1d.npz",
     "spectral radius (closed)": "golden localmax (LAPACK) to 1e-13 rel",
@@ -84,6 +88,7 @@ class Scheme:
     compat: str = "astrea"        # astrea: reference quirks | physical: solver axis = sweep axis
     category: str = ""            # solver_category when it differs from the name's (generic.py:241,266-269)
     av: bool = False              # artificial viscosity on the fluxes (1D PPM; the reference never applies it)
+    ppm_author: str = "mc"        # mc | c | ph (ppm.py:13; evolvers.py:17 always passes 'mc')
 
     @property
     def high_order(self):
@@ -393,6 +398,74 @@ def mc_extrapolant_limiter(wS, bc, wF_L, wF_R):
     return wL, wR
 
 
+def interface_limiter(wf, wm1, w0, wp1, wp2):
+    """limiters.py:53-79 (Colella et al. 2011 eq. 84-87; Peterson & Hammett
+    2008 eq. 3.33-3.34) for face values wf between cells w0 and wp1.  The
+    reference decides over the whole array at once (line 58): if no face
+    anywhere is a local extremum, every face keeps its value; otherwise every
+    face is replaced by the average of its two cells minus a sixth of the
+    limited curvature, which is non-zero only where the three curvature
+    estimates share one sign."""
+    if not np.any((wf - w0) * (wp1 - wf) < 0):
+        return wf
+    C = 5 / 4
+    dl = wm1 - 2 * w0 + wp1
+    dc = 3 * (w0 - 2 * wf + wp1)
+    dr = w0 - 2 * wp1 + wp2
+    same = (np.sign(dl) == np.sign(dr)) & (np.sign(dc) == np.sign(dr)) & (np.sign(dc) == np.sign(dl))
+    lim = np.sign(dc) * np.minimum(np.abs(dc), np.minimum(np.abs(C * dl), np.abs(C * dr)))
+    d2 = np.zeros_like(wf)
+    d2[same] = lim[same]
+    return .5 * (w0 + wp1) - d2 / 6
+
+
+def colella_extrapolant_limiter(wS, w, w2, wFp, wF_L, wF_R, author):
+    """limiters.py:144-201: the extrapolant limiter of PPM authors 'c'
+    (Colella et al. 2011 eq. 89-98) and 'ph' (Peterson & Hammett 2008 eq.
+    3.31-3.39).  ``w``/``w2`` are wS padded by 1/2 cells, ``wFp`` the face
+    values padded by 2.  The reference's global test (line 163) leaves the
+    face values unlimited when no cell or extrapolant anywhere is an
+    extremum; otherwise phi = D2_lim / D2 scales every cell, and D2_lim is
+    zero away from extrema (those cells become piecewise constant)."""
+    C = 5 / 4
+    dm, dp = wS - wF_L, wF_R - wS
+    cell_ext = dm * dp <= 0
+    ph = author == "ph"
+    if ph:
+        ext_ext = (w[:-2] - wS) * (wS - w[2:]) <= 0
+    else:
+        dfl, dfr = wF_L - wFp[:-4], wFp[4:] - wF_R
+        dsl, dsr = wS - w[:-2], w[2:] - wS
+        df = np.minimum(np.abs(dfl), np.abs(dfr))
+        ds = np.minimum(np.abs(dsl), np.abs(dsr))
+        ext_ext = ((df >= ds) & (dfl * dfr < 0)) | ((ds >= df) & (dsl * dsr < 0))
+    if not (np.any(cell_ext) or np.any(ext_ext)):
+        return wF_L, wF_R
+    d2 = 6 * (wF_L - 2 * wS + wF_R)
+    d2l = w2[:-4] - 2 * w[:-2] + wS
+    d2c = w[:-2] - 2 * wS + w[2:]
+    d2r = wS - 2 * w[2:] + w2[4:]
+    sg = [np.sign(x) for x in (d2, d2c, d2l, d2r)]
+    same = ((sg[0] == sg[1]) & (sg[0] == sg[2]) & (sg[0] == sg[3]) & (sg[1] == sg[2]) & (sg[1] == sg[3])
+            & (sg[2] == sg[3]))
+    lim = sg[0] * np.minimum(np.minimum(np.abs(d2), np.abs(C * d2c)), np.minimum(np.abs(C * d2l), np.abs(C * d2r)))
+    d2lim = np.zeros_like(wS)
+    m = cell_ext & same
+    d2lim[m] = lim[m]
+    if ph:
+        phi = gdiv(d2lim, d2)
+        return wS + phi * (wF_L - wS), wS + phi * (wF_R - wS)
+    m = ext_ext & same
+    d2lim[m] = lim[m]
+    phi = gdiv(d2lim, d2)
+    dl, dr = np.copy(dm), np.copy(dp)
+    m = np.abs(dm) > 2 * np.abs(dp)
+    dl[m] = 2 * dp[m]
+    m = np.abs(dp) > 2 * np.abs(dm)
+    dr[m] = 2 * dm[m]
+    return wS - phi * dl, wS + phi * dr
+
+
 def flattener_eta(wS, axis, bc, variant, delta=.33, z0=.75, z1=.85):
     """PPM slope flattener (schemes/ppm.py:111-134, Colella 1990).
 
@@ -559,11 +632,23 @@ def reconstruct(q, s, axis):
         w = w2[1:-1]
         wF = C7_12 * (wS + w[2:]) - C1_12 * (w[:-2] + w2[4:])
         wF_unlim = wF
-        if s.flattener != "off":
-            eta = flattener_eta(wS, axis, bc, s.flattener)
-            wF = wF * eta[..., None] + wS * (1 - eta)[..., None]
-        wFp = pad(wF, bc, 2)
-        wL, wR = mc_extrapolant_limiter(wS, bc, np.copy(wFp[1:-3]), np.copy(wFp[2:-2]))
+        if s.ppm_author == "ph":
+            # ppm.py:43-57: both faces of each cell interpolated and limited separately
+            wF_L = C7_12 * (w[:-2] + wS) - C1_12 * (w2[:-4] + w[2:])
+            lim = (interface_limiter(wF_L, w2[:-4], w[:-2], wS, w[2:]),
+                   interface_limiter(wF, w[:-2], wS, w[2:], w2[4:]))
+            wL, wR = colella_extrapolant_limiter(wS, w, w2, None, lim[0], lim[1], "ph")
+        elif s.ppm_author == "c":
+            # ppm.py:60-62, 69-71: the face values limited, then the extrapolants
+            wF = interface_limiter(wF, w[:-2], wS, w[2:], w2[4:])
+            wFp = pad(wF, bc, 2)
+            wL, wR = colella_extrapolant_limiter(wS, w, w2, wFp, np.copy(wFp[1:-3]), np.copy(wFp[2:-2]), "c")
+        else:
+            if s.flattener != "off":
+                eta = flattener_eta(wS, axis, bc, s.flattener)
+                wF = wF * eta[..., None] + wS * (1 - eta)[..., None]
+            wFp = pad(wF, bc, 2)
+            wL, wR = mc_extrapolant_limiter(wS, bc, np.copy(wFp[1:-3]), np.copy(wFp[2:-2]))
         d["wT_src"] = wF_unlim     # ppm.py:40-41 reads wF before the blend
     else:
         wL, wR = weno(wS, bc, weno_order(s.subgrid))

@@ -331,3 +331,46 @@ def test_artificial_viscosity_hook_is_diffusive():
     w[:, 1] = np.sin(2 * np.pi * x) * 0 + .3    # uniform flow: no compression anywhere
     q = O.initial_average(w, s0)
     assert np.array_equal(O.run(q, s0, 1e9, max_steps=5).grid, O.run(q, s1, 1e9, max_steps=5).grid)
+
+
+# ---- interface limiter and the other PPM authors (limiters.py:53-79, 144-201) ---
+@pytest.mark.parametrize("key", sorted(k[:-3] for k in PINS.files if k.startswith("iflim_") and k.endswith("_wS")))
+def test_interface_limiter_equals_reference(key):
+    """The reference's interface_limiter on random 1D hydro/MHD data and on
+    smooth monotone data, both boundaries: bit for bit."""
+    wS = PINS[key + "_wS"]
+    bc = key.split("_")[-1]
+    w2 = O.pad(wS, bc, 2)
+    w = w2[1:-1]
+    wF = O.C7_12 * (wS + w[2:]) - O.C1_12 * (w[:-2] + w2[4:])
+    assert np.array_equal(O.interface_limiter(wF, w[:-2], wS, w[2:], w2[4:]), PINS[key + "_out"])
+
+
+def test_interface_limiter_constant_data_keeps_its_faces():
+    """No face is a local extremum on constant data, so the reference's
+    global test keeps the face values (and the limited form would give the
+    same value there)."""
+    wS = np.full((16, 8), 1.5)
+    w2 = O.pad(wS, "edge", 2)
+    w = w2[1:-1]
+    wF = O.C7_12 * (wS + w[2:]) - O.C1_12 * (w[:-2] + w2[4:])
+    assert np.array_equal(O.interface_limiter(wF, w[:-2], wS, w[2:], w2[4:]), wF)
+
+
+@pytest.mark.parametrize("key", sorted(k[:-5] for k in PINS.files if k.startswith("ppm_") and k.endswith("_grid")))
+def test_ppm_authors_c_and_ph_equal_reference(key):
+    """schemes.ppm.run with author 'c' (Colella et al. 2011: interface
+    limiter, then the Colella extrapolant limiter) and 'ph' (Peterson &
+    Hammett 2008: both faces interpolated and limited, their extrapolant
+    limiter), 1D hydro and MHD, both boundaries: the interface states equal
+    the reference's bit for bit."""
+    _, author, kind, bc = key.split("_")
+    s = O.Scheme(dimension=1, subgrid="ppm", boundary=bc, gamma=5 / 3 if kind == "mhd" else 1.4, dx=1 / 32,
+                 ppm_author=author)
+    d = O.reconstruct(PINS[key + "_grid"], s, 0)
+    assert np.array_equal(d["wFs"][0], PINS[key + "_wplus"])
+    assert np.array_equal(d["wFs"][1], PINS[key + "_wminus"])
+    # and they differ from the 'mc' states the reference's own option path runs
+    d_mc = O.reconstruct(PINS[key + "_grid"], O.Scheme(dimension=1, subgrid="ppm", boundary=bc, dx=1 / 32,
+                                                       gamma=s.gamma), 0)
+    assert not np.array_equal(d["wFs"][0], d_mc["wFs"][0])
