@@ -16,6 +16,7 @@ AH_EDGE, AH_WRAP = 0, 1
 AH_PCM, AH_PLM, AH_PPM, AH_WENO3, AH_WENO5, AH_WENO7 = range(6)
 AH_MINMOD, AH_VANLEER, AH_OSPRE, AH_VANALBADA1, AH_KOREN, AH_SUPERBEE = range(6)
 AH_FLAT_OFF, AH_FLAT_ASTREA, AH_FLAT_COLELLA90 = range(3)
+AH_PPM_MC, AH_PPM_C, AH_PPM_PH = range(3)
 AH_LLF, AH_HLLC, AH_HLLD, AH_DOTS, AH_ES, AH_LW = range(6)
 AH_EULER, AH_RK4, AH_SSPRK22, AH_SSPRK33, AH_SSPRK43, AH_SSPRK53, AH_SSPRK54, AH_SSPRK104 = range(8)
 AH_FP64, AH_SPLIT = 0, 1
@@ -39,7 +40,7 @@ class AhScheme(ctypes.Structure):
                 ("flattener", ctypes.c_int), ("solver", ctypes.c_int), ("rk", ctypes.c_int),
                 ("precision", ctypes.c_int), ("varmask", ctypes.c_int), ("magnetic_2d", ctypes.c_int),
                 ("tile_s", ctypes.c_int), ("tile_t", ctypes.c_int), ("compat", ctypes.c_int),
-                ("av", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("av", ctypes.c_int), ("ppm_author", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
 
 
 class AhProblem(ctypes.Structure):

@@ -29,7 +29,7 @@ def plan_for(sim_variables, grids, precision=None):
     mask = infer_varmask(grids, dims, sv.magnetic_2d)
     return Plan(dims, sv.subgrid, sv.solver, sv.timestep, problems, limiter=sv.limiter, flattener=sv.flattener,
                 precision=precision or sv.precision_mode, varmask=mask, magnetic_2d=sv.magnetic_2d,
-                solver_category=sv.solver_category, av=sv.av)
+                solver_category=sv.solver_category, av=sv.av, ppm_author=sv.ppm_author)
 
 
 def run_saving(plan, save, t_end, i=0):

@@ -56,6 +56,15 @@ std::vector<Stage> rk_program(int rk) {
     return {};
 }
 
+// PPM authors c/ph (ppm.py:43-62): 1D hydrodynamics (whose zero components
+// make the extrapolant limiter's grid-wide test always true), FP64, without
+// the flattener and artificial viscosity the reference ties to author 'mc'
+bool author_supported(const ah_scheme* sch) {
+    return (sch->ppm_author == AH_PPM_C || sch->ppm_author == AH_PPM_PH) && sch->dims == 1 &&
+           sch->recon == AH_PPM && sch->varmask == AH_VARS_H1 && sch->precision == AH_FP64 &&
+           sch->flattener == AH_FLAT_OFF && !sch->av && !sch->magnetic_2d;
+}
+
 int popcount8(int m) {
     int n = 0;
     for (int c = 0; c < 8; ++c) n += (m >> c) & 1;
@@ -233,7 +242,9 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     for (int c = 0, k = 0; c < 8; ++c)
         if ((sch->varmask >> c) & 1) p->varidx[k++] = c;
     bool ok;
-    if (sch->precision == AH_FP64)
+    if (sch->ppm_author != AH_PPM_MC)
+        ok = author_supported(sch) && kernels_1d_author_fp64(sch->ppm_author, sch->solver, &p->ks);
+    else if (sch->precision == AH_FP64)
         ok = sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks)
                             : kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks);
     else
@@ -482,10 +493,11 @@ static int blocks_for(const ah_plan* p, const TileList& L) { return std::min<int
 // L of the stage input X for every active problem (evolve_space + compute_L)
 static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, int parity_override,
                          double* flux0, double* flux1, hipStream_t s, bool fuse33 = false,
-                         const double* xl = nullptr) {
+                         const double* xl = nullptr, int stage_idx = 0) {
     SweepArgs a = base_args(p);
     a.X = X;
     a.xl = xl;
+    a.stage_idx = stage_idx;
     a.stage0 = stage0;
     a.parity_override = parity_override;
     if (fuse33) {
@@ -499,6 +511,12 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.ntiles_dev = p->tl[0].d_count;
         a.flux_out = flux0;
         a.flux_plane = p->probs[0].n0 + 1;
+        if (p->sch.ppm_author != AH_PPM_MC) {
+            // the interface-limiter tests of this stage over every cell first
+            SweepArgs f = a;
+            f.flags_only = 1;
+            p->ks.sweep1d(f, blocks_for(p, p->tl[0]), s);
+        }
         ProfScope ps_(p, AH_K_SWEEP1D, s);
         p->ks.sweep1d(a, blocks_for(p, p->tl[0]), s);
         return;
@@ -602,9 +620,9 @@ static void launch_step(ah_plan* p, hipStream_t s) {
         // combination launch it saves -- 0.926 -> 0.946 ms per shear-layer step)
         const bool k1_live = (S.op == OP_33_0 || S.op == OP_33_1) && !mhd && p->sch.dims == 1;
         if (fuse33 && k1_live)
-            launch_space(p, p->regs[0], p->Lbuf, 0, -1, nullptr, nullptr, s, true, p->Lbuf);
+            launch_space(p, p->regs[0], p->Lbuf, 0, -1, nullptr, nullptr, s, true, p->Lbuf, (int)st);
         else
-            launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s, fuse33);
+            launch_space(p, p->regs[S.in], p->Lbuf, st == 0, -1, nullptr, nullptr, s, fuse33, nullptr, (int)st);
         MhdArgs m;
         if (mhd) {
             // corner EMF from both sweeps' face values; it replaces the B rows of L
@@ -663,6 +681,7 @@ struct ProbeState {
         tmp.eig[0] = tmp.eig[1] = 0ull;
         tmp.parity = parity;
         tmp.dt = dt;
+        tmp.aflags = 0ull;
         HIPCHK(hipMemcpyAsync(p->d_ps, &tmp, sizeof(ProbState), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
         return AH_OK;
@@ -1047,7 +1066,7 @@ int ah_evolve_time(ah_plan* p, int prob, int parity, double* grid, const double*
     for (size_t st = 0; st < p->prog.size(); ++st) {
         const Stage& S = p->prog[st];
         if (st > 0) {
-            launch_space(p, p->regs[S.in], p->Lbuf, 0, -1, nullptr, nullptr, s);
+            launch_space(p, p->regs[S.in], p->Lbuf, 0, -1, nullptr, nullptr, s, false, nullptr, (int)st);
             if (mhd) {
                 MhdArgs m = mhd_args(p);
                 p->ks.trecon[0](m, blocks_for(p, p->tl[6]), s);
@@ -1132,6 +1151,7 @@ __global__ void static_init_kernel(ProbDesc* pd, ProbDesc desc, ProbState* ps, i
         s.eig[0] = s.eig[1] = 0ull;
         s.fail_flag = 0;
         s.ckpt = 0;
+        s.aflags = 0ull;
         *ps = s;
     }
     const TileSpec specs[4] = {t0, t1, t2, t3};
@@ -1155,11 +1175,14 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
         return AH_ERR_INVALID;
     if (sch->magnetic_2d) return AH_ERR_UNSUPPORTED;
     if (sch->av && (sch->dims != 1 || sch->recon != AH_PPM)) return AH_ERR_UNSUPPORTED;
+    if (sch->ppm_author != AH_PPM_MC && !author_supported(sch)) return AH_ERR_UNSUPPORTED;
     const int n0 = q->n0, n1 = sch->dims == 1 ? 1 : q->n1;
     if (n0 <= 0 || n1 <= 0 || q->dx <= 0 || q->checkpoints <= 0 || (q->boundary != AH_EDGE && q->boundary != AH_WRAP))
         return AH_ERR_INVALID;
     KernelSet ks;
-    bool ok = sch->precision == AH_FP64
+    bool ok = sch->ppm_author != AH_PPM_MC
+                  ? kernels_1d_author_fp64(sch->ppm_author, sch->solver, &ks)
+                  : sch->precision == AH_FP64
                   ? (sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &ks)
                                     : kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &ks))
                   : (sch->dims == 1 ? kernels_1d_fp32(sch->varmask, sch->recon, sch->solver, &ks)
@@ -1262,6 +1285,11 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
         a.ntiles_dev = spec[0].count;
         a.flux_out = flux0;
         a.flux_plane = n0 + 1;
+        if (sch->ppm_author != AH_PPM_MC) {
+            SweepArgs f = a;
+            f.flags_only = 1;
+            ks.sweep1d(f, blocks(0), s);
+        }
         ks.sweep1d(a, blocks(0), s);
     } else {
         a.wL = wL;

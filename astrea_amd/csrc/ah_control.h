@@ -40,6 +40,7 @@ __global__ void advance_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
     if (p >= nprob) return;
     ProbState& s = ps[p];
     if (!s.active) return;
+    s.aflags = 0ull;
     const ProbDesc d = pd[p];
     if (s.fail_flag) {
         // the reference raises inside this step (LAPACK on a non-finite
@@ -65,6 +66,7 @@ __global__ void finish_kernel(const ProbDesc* pd, ProbState* ps, int nprob, int 
     if (p >= nprob) return;
     ProbState& s = ps[p];
     if (!s.active) return;
+    s.aflags = 0ull;
     const ProbDesc d = pd[p];
     const double e0 = __longlong_as_double((long long)s.eig[0]);
     const double e1 = dims == 2 ? __longlong_as_double((long long)s.eig[1]) : e0;
@@ -106,6 +108,7 @@ __global__ void reset_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
     s.eig[0] = s.eig[1] = 0ull;
     s.fail_flag = 0;
     s.ckpt = 0;
+    s.aflags = 0ull;
     const ProbDesc d = pd[p];
     s.active = (0.0 <= d.t_end) && !(0.0 == d.t_end) && d.max_steps != 0;
     ps[p] = s;

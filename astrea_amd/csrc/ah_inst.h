@@ -5,14 +5,14 @@
 
 namespace ah {
 
-template <class R, int MASK, int RECON, int SOLVER, bool SPLIT>
+template <class R, int MASK, int RECON, int SOLVER, bool SPLIT, int AUTH = AUTH_MC>
 inline void set1d(KernelSet* ks) {
     // cells per tile: the widest per-stage window (the q window, TS + 13)
     // fits one pass of the 256 threads, so no stage runs a second, nearly
     // empty round while the other waves wait at the barrier
     constexpr int TS = VS<MASK>::NV > 4 ? 128 : 256 - 13;
     ks->sweep1d = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS, SPLIT>, a, nb, s);
+        launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS, SPLIT, AUTH>, a, nb, s);
     };
     ks->ts1 = TS;
     ks->real_bytes = sizeof(R);

@@ -30,6 +30,10 @@ struct ProbState {
     unsigned long long eig[2];  // bit pattern of max spectral radius per axis (stage 0)
     int fail_flag;
     int ckpt;  // 1 when this step's dt was clamped to the next checkpoint (astrea.py:74-78)
+    // PPM authors c/ph: the reference's grid-wide interface-limiter tests of
+    // this step, two bits per stage (bit 2k: some face wF is a local extremum,
+    // bit 2k+1: some left-face value wF_L is), cleared when a step ends
+    unsigned long long aflags;
 };
 
 struct Tile {
@@ -78,6 +82,8 @@ struct SweepArgs {
     int last;             // 2D: this flux launch finishes L
     int compat;           // 0: reference solver axis (enumeration index), 1: physical axis
     int av;               // 1D PPM: add the artificial-viscosity flux alpha nu (q_k - q_{k-1})
+    int stage_idx;        // stage of the step (PPM authors c/ph read that stage's limiter tests)
+    int flags_only;       // 1D PPM authors c/ph: only record this stage's interface-limiter tests
     // 2D second sweep of the middle SSPRK(3,3) stage: write K2 = (3U + K1 + dt L) / 4
     // (evolvers.py:172) in place of L, in the combine kernel's operation order
     double* fuse_k2;      // null: write L
@@ -266,6 +272,124 @@ __device__ inline void recon_cell(const Acc& W, int ce, int limiter, int flatten
                                    [&](int mm) { return flat_chi_at<R, AX>(W, flattener, mm); });
 }
 
+// ---- PPM authors 'c' and 'ph' (ppm.py:43-62, 69-71) ----------------------------
+// Colella et al. 2011 and Peterson & Hammett 2008: not reachable from the
+// reference's options (evolvers.py:17 passes author 'mc'), built as the
+// ppm_author option for 1D hydrodynamics.
+constexpr int AUTH_MC = 0, AUTH_C = 1, AUTH_PH = 2;
+
+// limiters.py:53-79 for one face value wf between cells w0 and wp1; `any`
+// is the reference's grid-wide test (line 58): some face of the sweep array
+// is a local extremum, (wf - w0)(wp1 - wf) < 0
+template <class R>
+__device__ inline R iface_limit(bool any, R wf, R wm1, R w0, R wp1, R wp2) {
+    if (!any) return wf;
+    constexpr double C = 5.0 / 4.0;
+    const R dl = wm1 - R(2) * w0 + wp1;
+    const R dc = R(3) * (w0 - R(2) * wf + wp1);
+    const R dr = w0 - R(2) * wp1 + wp2;
+    const R sl = nsign(dl), sc = nsign(dc), sr = nsign(dr);
+    R d2 = R(0);
+    if (sl == sr && sc == sr && sc == sl)
+        d2 = sc * nmin(ah_fabs(dc), nmin(ah_fabs(R(C) * dl), ah_fabs(R(C) * dr)));
+    return R(0.5) * (w0 + wp1) - d2 / R(6);
+}
+
+template <class R>
+__device__ inline bool iface_extremum(R wf, R w0, R wp1) { return (wf - w0) * (wp1 - wf) < R(0); }
+
+// One cell's extrapolants for author c / ph (limiters.py:144-201), component
+// by component.  The reference's grid-wide branch test of line 163 is true on
+// every array this build runs these authors on: its 8-component arrays carry
+// the components a 1D hydrodynamic state keeps at zero, whose
+// (wS - wF_L)(wF_R - wS) = 0 <= 0 marks every cell as an extremum.
+// anyR / anyL: the interface-limiter tests of this stage for the face values
+// wF (faces i+1/2) and, for 'ph', the left-face values wF_L.
+template <class R, int MASK, int AUTH, class Acc>
+__device__ inline void author_cell(const Acc& W, int ce, bool anyR, bool anyL, V8<R>& wL, V8<R>& wR) {
+    using V = VS<MASK>;
+    constexpr double C = 5.0 / 4.0;
+    wL = zero8<R, MASK>();
+    wR = zero8<R, MASK>();
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (!V::live(c)) continue;
+        auto w = [&](int m) { return W(m).c[c]; };
+        // face m+1/2 (ppm.py:38) at face index e(m), limited (limiters.py:53-79)
+        auto face = [&](int m) {
+            const int me = W.e(m);
+            const R f = R(K7_12) * (w(me) + w(me + 1)) - R(K1_12) * (w(me - 1) + w(me + 2));
+            return iface_limit<R>(anyR, f, w(me - 1), w(me), w(me + 1), w(me + 2));
+        };
+        const R s0 = w(ce), sm = w(ce - 1), sp = w(ce + 1), sm2 = w(ce - 2), sp2 = w(ce + 2);
+        R fL, fR;
+        if constexpr (AUTH == AUTH_PH) {
+            // ppm.py:51-57: the cell's own left-face interpolant and its face i+1/2
+            const R fl0 = R(K7_12) * (sm + s0) - R(K1_12) * (sm2 + sp);
+            fL = iface_limit<R>(anyL, fl0, sm2, sm, s0, sp);
+            const R fr0 = R(K7_12) * (s0 + sp) - R(K1_12) * (sm + sp2);
+            fR = iface_limit<R>(anyR, fr0, sm, s0, sp, sp2);
+        } else {
+            fL = face(ce - 1);
+            fR = face(ce);
+        }
+        const R dm = s0 - fL, dp = fR - s0;
+        const bool cell_ext = dm * dp <= R(0);
+        bool ext_ext;
+        if constexpr (AUTH == AUTH_PH) {
+            ext_ext = (sm - s0) * (s0 - sp) <= R(0);
+        } else {
+            // limiters.py:155-161 (wF_pad2[:-4], wF_pad2[4:]: faces i-3/2 and i+5/2)
+            const R dfl = fL - face(ce - 2), dfr = face(ce + 2) - fR;
+            const R dsl = s0 - sm, dsr = sp - s0;
+            const R df = nmin(ah_fabs(dfl), ah_fabs(dfr)), ds = nmin(ah_fabs(dsl), ah_fabs(dsr));
+            ext_ext = ((df >= ds) && (dfl * dfr < R(0))) || ((ds >= df) && (dsl * dsr < R(0)));
+        }
+        const R d2 = R(6) * (fL - R(2) * s0 + fR);
+        const R d2l = sm2 - R(2) * sm + s0;
+        const R d2c = sm - R(2) * s0 + sp;
+        const R d2r = s0 - R(2) * sp + sp2;
+        const R g0 = nsign(d2), gc = nsign(d2c), gl = nsign(d2l), gr = nsign(d2r);
+        const bool same = g0 == gc && g0 == gl && g0 == gr && gc == gl && gc == gr && gl == gr;
+        const R lim = g0 * nmin(nmin(ah_fabs(d2), ah_fabs(R(C) * d2c)), nmin(ah_fabs(R(C) * d2l), ah_fabs(R(C) * d2r)));
+        R d2lim = R(0);
+        if (cell_ext && same) d2lim = lim;
+        if constexpr (AUTH == AUTH_PH) {
+            const R phi = gdiv(d2lim, d2);
+            wL.c[c] = s0 + phi * (fL - s0);
+            wR.c[c] = s0 + phi * (fR - s0);
+        } else {
+            if (ext_ext && same) d2lim = lim;
+            const R phi = gdiv(d2lim, d2);
+            const R dl = ah_fabs(dm) > R(2) * ah_fabs(dp) ? R(2) * dp : dm;
+            const R dr = ah_fabs(dp) > R(2) * ah_fabs(dm) ? R(2) * dm : dp;
+            wL.c[c] = s0 - phi * dl;
+            wR.c[c] = s0 + phi * dr;
+        }
+    }
+}
+
+// this stage's interface-limiter tests of one cell's faces: bit 0 for its
+// face i+1/2 (wF), bit 1 for its left-face value wF_L (author 'ph')
+template <class R, int MASK, int AUTH, class Acc>
+__device__ inline unsigned author_face_tests(const Acc& W, int ce) {
+    using V = VS<MASK>;
+    unsigned bits = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (!V::live(c)) continue;
+        auto w = [&](int m) { return W(m).c[c]; };
+        const R s0 = w(ce), sm = w(ce - 1), sp = w(ce + 1), sm2 = w(ce - 2), sp2 = w(ce + 2);
+        const R fr0 = R(K7_12) * (s0 + sp) - R(K1_12) * (sm + sp2);
+        if (iface_extremum<R>(fr0, s0, sp)) bits |= 1u;
+        if constexpr (AUTH == AUTH_PH) {
+            const R fl0 = R(K7_12) * (sm + s0) - R(K1_12) * (sm2 + sp);
+            if (iface_extremum<R>(fl0, sm, s0)) bits |= 2u;
+        }
+    }
+    return bits;
+}
+
 template <int MASK, bool SPLIT>
 __device__ inline V8<double> c2p(const V8<double>& q, double gamma) {
     if constexpr (SPLIT) return cons_to_prim_split<MASK>(q, gamma);
@@ -358,7 +482,7 @@ struct LineAcc {
     }
 };
 
-template <class R, int MASK, int RECON, int SOLVER, int TS, bool SPLIT = false>
+template <class R, int MASK, int RECON, int SOLVER, int TS, bool SPLIT = false, int AUTH = AUTH_MC>
 __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     static_assert(!SPLIT || sizeof(R) == 8, "split path keeps FP64 values");
     using V = VS<MASK>;
@@ -378,6 +502,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
     __shared__ R rad[NR];
     __shared__ R fl[NV][NF];
     __shared__ double red[4];
+    __shared__ unsigned abits;  // PPM authors c/ph: the tile's interface-limiter tests
     // PPM flattener: chi~ at positions [s0-3, s0+TS+2]
     constexpr int NCHI = TS + 6;
     __shared__ R chi_s[RECON == PPM ? NCHI : 1];
@@ -463,14 +588,51 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
             return v;
         }
     } W{ws, bw, N, bc};
+    if constexpr (RECON == PPM && AUTH != AUTH_MC) {
+        if (a.flags_only) {
+            // the reference's grid-wide interface-limiter tests of this stage
+            // (limiters.py:58) over the tile's own cells
+            if (threadIdx.x == 0) abits = 0u;
+            __syncthreads();
+            unsigned bits = 0;
+            for (int i = threadIdx.x; i < TS; i += blockDim.x) {
+                const int ce = s0 + i;
+                if (ce < N) bits |= author_face_tests<R, MASK, AUTH>(W, ce);
+            }
+            if (bits) atomicOr(&abits, bits);
+            __syncthreads();
+            if (threadIdx.x == 0 && abits) atomicOr(&ps->aflags, (unsigned long long)abits << (2 * a.stage_idx));
+            __syncthreads();  // abits and the LDS windows are reused by the next tile
+            continue;
+        }
+    }
     // C: reconstruction at cells [s0-1, s0+TS+2); their +-4 stencils cover
     // the window [s0-5, s0+TS+6), so tiles with that window inside the domain
     // (and every periodic tile) skip the boundary map
     if constexpr (RECON != PCM) {
         const bool clampC = bc == EDGE && !(s0 >= HW_L && s0 + TS + HW_R <= N);
+        bool anyR = false, anyL = false;
+        if constexpr (AUTH != AUTH_MC) {
+            const unsigned long long af = ld_const(&ps->aflags) >> (2 * a.stage_idx);
+            anyR = (af & 1ull) != 0;
+            anyL = (af & 2ull) != 0;
+        }
         auto recon_cells = [&](auto clamp_tag) {
             constexpr bool CL = decltype(clamp_tag)::value;
             const LineAcc<R, MASK, NW, CL> Wc{ws, bw, N};
+            if constexpr (RECON == PPM && AUTH != AUTH_MC) {
+                for (int i = threadIdx.x; i < NC; i += blockDim.x) {
+                    V8<R> L, Rr;
+                    author_cell<R, MASK, AUTH>(Wc, Wc.e(bcl + i), anyR, anyL, L, Rr);
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (V::live(c)) {
+                            wl[V::slot(c)][i] = L.c[c];
+                            wr[V::slot(c)][i] = Rr.c[c];
+                        }
+                }
+                return;
+            }
             if constexpr (RECON == PPM) {
                 // the cells' face etas read chi~ at me-1, me, me+1 with
                 // me = e(ce-1), e(ce): one evaluation per position

@@ -9,7 +9,8 @@ schemes/plm.py:66), ``flattener`` (PPM slope flattener: off | astrea |
 colella90), ``precision_mode`` (fp64 | split), ``max_steps`` and ``av``
 (artificial viscosity on the 1D PPM fluxes; the reference computes it only
 with a flag no option sets and never applies it, so the default False is its
-behaviour).
+behaviour) and ``ppm_author`` (mc | c | ph, schemes/ppm.py:13; the reference's
+evolve_space always passes 'mc').
 """
 import itertools
 from collections import namedtuple
@@ -58,14 +59,17 @@ ACCEPTED = {
     "limiter": {"": ["minmod", "vanleer", "ospre", "vanalbada1", "koren", "superbee"]},
     "flattener": {"": ["off", "astrea", "colella90"]},
     "precision_mode": {"": ["fp64", "split"]},
+    "ppm_author": {"": ["mc", "c", "ph"]},
 }
 # static/.db.json "default" record
 DEFAULTS = {"config": "sod", "dimension": 1, "precision": "float64", "subgrid": "pcm", "timestep": "euler",
-            "solver": "lf", "run_type": "single", "limiter": "minmod", "flattener": "off", "precision_mode": "fp64"}
+            "solver": "lf", "run_type": "single", "limiter": "minmod", "flattener": "off", "precision_mode": "fp64",
+            "ppm_author": "mc"}
 
 SimVariables = namedtuple("SimVariables", [
     "config", "cells", "cfl", "gamma", "dimension", "precision", "subgrid", "timestep", "solver", "run_type",
-    "checkpoints", "limiter", "flattener", "precision_mode", "max_steps", "av", "config_category", "solver_category",
+    "checkpoints", "limiter", "flattener", "precision_mode", "max_steps", "av", "ppm_author", "config_category",
+    "solver_category",
     "magnetic_2d", "permutations", "ortho_axis", "boundary", "dx", "t_end",
     "start_pos", "end_pos", "shock_pos", "misc", "initial_left", "initial_right"])
 

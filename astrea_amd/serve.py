@@ -206,15 +206,22 @@ class Server:
         shrink = max((sum(m) - min(m) for m in members.values() if len(m) > 1), default=0)
         return max(peak, total + shrink)
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, progress=None):
+        """Serve ``requests`` (arrival times relative to the start; with
+        ``realtime`` False all are queued at once).  ``progress``: seconds
+        between progress lines on stderr (None: silent)."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         finished = []
         rebuilds = 0
+        last = 0.0
         while pending or waiting or any(g.reqs for g in self.groups.values()):
             tn = now()
+            if progress is not None and tn - last >= progress:
+                last = tn
+                _progress_line("served", tn, len(finished), len(requests))
             while pending and (pending[0].arrival <= tn or not realtime):
                 waiting.append(pending.popleft())
             extra, admitted = {}, []
@@ -260,13 +267,22 @@ class Server:
         return _summary(finished, now(), rebuilds)
 
 
+def _progress_line(what, t, done, total):
+    import sys
+    print(f"[serve] {what}: {done}/{total} requests done at {t:.0f} s", file=sys.stderr, flush=True)
+
+
 def run_fifo(requests, precision="fp64", compat="physical", cfl=.4, chunk=64, realtime=True, keep_results=False,
-             device="cuda:0"):
+             device="cuda:0", progress=None):
     """The plain loop: one request at a time in arrival order."""
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0
     done = []
+    last = 0.0
     for r in sorted(requests, key=lambda r: r.arrival):
+        if progress is not None and now() - last >= progress:
+            last = now()
+            _progress_line("plain", last, len(done), len(requests))
         if realtime and r.arrival > now():
             time.sleep(r.arrival - now())
         r.admitted_at = now()

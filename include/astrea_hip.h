@@ -51,6 +51,11 @@ enum ah_boundary { AH_EDGE = 0, AH_WRAP = 1 };                /* "edge" (outlet)
 enum ah_recon { AH_PCM = 0, AH_PLM = 1, AH_PPM = 2, AH_WENO3 = 3, AH_WENO5 = 4, AH_WENO7 = 5 };
 enum ah_limiter { AH_MINMOD = 0, AH_VANLEER = 1, AH_OSPRE = 2, AH_VANALBADA1 = 3, AH_KOREN = 4, AH_SUPERBEE = 5 };
 enum ah_flattener { AH_FLAT_OFF = 0, AH_FLAT_ASTREA = 1, AH_FLAT_COLELLA90 = 2 };
+/* PPM authors (schemes/ppm.py:43-71): McCorquodale & Colella 2011 ('mc', the
+   reference's only reachable one), Colella et al. 2011 ('c': interface limiter
+   limiters.py:53-79, then the extrapolant limiter limiters.py:144-201) and
+   Peterson & Hammett 2008 ('ph') */
+enum ah_ppm_author { AH_PPM_MC = 0, AH_PPM_C = 1, AH_PPM_PH = 2 };
 /* solvers.py:69-75 (LLF), 92-138 (HLLC), 142-232 (HLLD), 236-284 (Osher-Solomon
    with the Dumbser-Toro path integral; FP64, 1D hydro/MHD and 2D hydro),
    288-391 (entropy-stable as the reference writes it; FP64, 1D, all 8 variables),
@@ -96,7 +101,10 @@ typedef struct ah_scheme {
     int av;          /* 1: artificial viscosity on the fluxes (1D PPM only; schemes/ppm.py:138-170,
                         whose output the reference computes but never applies: 0 is the
                         reference-equivalent setting) */
-    int reserved[3];
+    int ppm_author;  /* ah_ppm_author (ppm.py:13; the reference's evolve_space always passes 'mc',
+                        evolvers.py:17); c / ph: 1D hydrodynamics in FP64 without flattener or
+                        artificial viscosity */
+    int reserved[2];
 } ah_scheme;
 
 typedef struct ah_problem {

@@ -388,3 +388,49 @@ def test_artificial_viscosity_refused_outside_1d_ppm():
         prob = dict(n0=16, n1=16, boundary="wrap", dx=1 / 16, gamma=1.4, cfl=.4, t_end=1.)
         with pytest.raises(L.AstreaHipError, match="not implemented"):
             Plan(dims, sg, "hllc", "ssprk(3,3)", [prob], varmask=0x13 if dims == 1 else 0x17, av=True)
+
+
+# ---- PPM authors c / ph (ppm.py:43-71, limiters.py:53-79, 144-201), opt-in ------
+@pytest.mark.parametrize("author", ["c", "ph"])
+@pytest.mark.parametrize("bc,solver", [("edge", "hllc"), ("wrap", "llf")])
+def test_ppm_author_trajectory(author, bc, solver):
+    """1D PPM with author 'c' or 'ph' (the grid-wide interface-limiter tests
+    recorded by a first pass over the grid each stage): evolve_space fluxes
+    and 20 steps of a Riemann problem equal the oracle, whose interface
+    states for these authors are pinned to the reference's ppm.run."""
+    from astrea_amd.plan import Plan
+    n = 300
+    x = (np.arange(n) + .5) / n
+    w = np.zeros((n, 8))
+    w[:] = [.3, -.4, 0, 0, .2, 0, 0, 0]
+    w[x <= .5] = [2.5, .7, 0, 0, 4., 0, 0, 0]
+    w[:, 0] *= 1 + .1 * np.sin(6 * np.pi * x)
+    s = O.Scheme(dimension=1, subgrid="ppm", solver=solver, boundary=bc, gamma=1.4, cfl=.4, dx=1 / n,
+                 ppm_author=author)
+    q0 = O.initial_average(w, s)
+    _say(f"[gpu] ppm author {author} {bc} {solver}")
+    prob = dict(n0=n, boundary=bc, dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9, max_steps=20)
+    plan = Plan(1, "ppm", solver, "ssprk(3,3)", [prob], varmask=0x13, ppm_author=author)
+    plan.load([q0])
+    fl = plan.evolve_space(0)
+    assert _scaled_err(fl[0]["flux"], O.evolve_space(q0, s)[0]["flux"]) <= 1e-14
+    plan.reset()
+    st = plan.run(32)[0]
+    out = plan.unpack(0)
+    plan.close()
+    ref = O.run(q0, s, 1e9, max_steps=20)
+    assert st["steps"] == ref.steps == 20 and not st["failed"]
+    e = float(np.max(rel_l1(out, ref.grid)))
+    _say(f"[gpu]   rel L1 vs oracle {e:.1e}")
+    assert e <= 1e-12
+    assert not np.array_equal(ref.grid, O.run(q0, replace(s, ppm_author="mc"), 1e9, max_steps=20).grid)
+
+
+def test_ppm_author_refused_outside_1d_hydro():
+    from astrea_amd import _lib as L
+    from astrea_amd.plan import Plan
+    prob = dict(n0=16, n1=16, boundary="wrap", dx=1 / 16, gamma=1.4, cfl=.4, t_end=1.)
+    with pytest.raises(L.AstreaHipError, match="not implemented"):
+        Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x17, ppm_author="c")
+    with pytest.raises(L.AstreaHipError, match="not implemented"):
+        Plan(1, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x13, ppm_author="ph", flattener="astrea")
