@@ -216,6 +216,7 @@ class Server:
         now = lambda: time.perf_counter() - t0
         finished = []
         rebuilds = 0
+        rebuild_s = 0.0
         last = 0.0
         while pending or waiting or any(g.reqs for g in self.groups.values()):
             tn = now()
@@ -251,7 +252,9 @@ class Server:
                 g.stale = True
             for g in self.groups.values():
                 if g.stale:
+                    r0 = time.perf_counter()
                     g.rebuild()
+                    rebuild_s += time.perf_counter() - r0
                     rebuilds += 1
             if not any(g.reqs for g in self.groups.values()):
                 if pending:
@@ -264,7 +267,9 @@ class Server:
         for g in self.groups.values():
             if g.plan:
                 g.plan.close()
-        return _summary(finished, now(), rebuilds)
+        out = _summary(finished, now(), rebuilds)
+        out["rebuild_s"] = rebuild_s
+        return out
 
 
 def _progress_line(what, t, done, total):

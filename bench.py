@@ -353,6 +353,7 @@ def serve_leg(args):
                     "(physical axis), SSPRK(3,3), CFL 0.4, t_end 0.25, edge; continuous batching, FP64",
         "saturated": {"plain_fp64_zu_per_s": plain["zu_per_s"], "served_fp64_zu_per_s": srv["zu_per_s"],
                       "speedup_vs_plain": srv["zu_per_s"] / plain["zu_per_s"], "rebuilds": srv["rebuilds"],
+                      "rebuild_s": srv["rebuild_s"], "makespan_s": srv["seconds"],
                       "l1_vs_plain_max": max(errs) if errs else None, "compared": len(errs),
                       "step_or_failure_mismatch": mism, "failed": srv["failed"]},
         "poisson": {"rate_per_s": args.rate, "plain_p50_s": plain_p["p50"], "plain_p99_s": plain_p["p99"],
@@ -444,9 +445,9 @@ def bench_serve(args):
         # each take the whole arrival window, so they run in separate calls)
         reqs = requests(True)
         if args.serve_only == "plain_fp64":
-            r = serve.run_fifo(reqs, realtime=True)
+            r = serve.run_fifo(reqs, realtime=True, progress=30)
         else:
-            r = serve.Server(precision="fp64").run(reqs, realtime=True)
+            r = serve.Server(precision="fp64").run(reqs, realtime=True, progress=30)
         r.pop("done")
         return {"metric": METRIC, "value": r["zu_per_s"], "unit": "zone-updates/s", "n_gpus": 1, "steps": None,
                 "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
