@@ -131,6 +131,9 @@ print(json.dumps(dict(seconds=dt, zu=r.steps * {n} * {n})))
     r = run(1, cpus[0])
     if r is None:
         return None
+    # the GPU box gives one GPU's process a share of 16 host cores (more CPUs may be
+    # visible); numpy's elementwise work is single-threaded, only the LAPACK calls
+    # use the extra BLAS threads
     nall = min(len(cpus), 16)
     r_all = run(nall, None)
     cpu = "unknown"

@@ -68,6 +68,11 @@ enum ah_rk {
     AH_EULER = 0, AH_RK4 = 1, AH_SSPRK22 = 2, AH_SSPRK33 = 3, AH_SSPRK43 = 4,
     AH_SSPRK53 = 5, AH_SSPRK54 = 6, AH_SSPRK104 = 7
 };
+/* AH_FP64: every operation in FP64, in the reference's order (bitwise the numpy
+ * reference).  AH_SPLIT: the Riemann solvers' coefficient algebra (wave-speed
+ * estimates, star-region factors), the spectral radius and the reciprocal of the
+ * primitive recovery in FP32; states, interface states, reconstruction, fluxes,
+ * stage combinations and the dt reduction in FP64.  Not bitwise. */
 enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };
 /* AH_COMPAT_ASTREA reproduces the reference's Riemann-solver axis argument:
  * the enumeration index of the sweep, which swaps the normal velocity of
@@ -77,7 +82,7 @@ enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };
  * i.e. when all waves move right (solvers.py:133).  AH_COMPAT_PHYSICAL passes
  * the physical sweep axis, writes the star momentum at that axis and returns
  * the upwind F(w-) there. */
-enum ah_compat { AH_COMPAT_ASTREA = 0, AH_COMPAT_PHYSICAL = 1 };  /* SPLIT: FP32 reconstruction/limiter/Riemann, FP64 state */
+enum ah_compat { AH_COMPAT_ASTREA = 0, AH_COMPAT_PHYSICAL = 1 };
 enum ah_varmask {
     AH_VARS_H1 = 0x13,  /* rho, rho*vx, E          : 1D hydrodynamics */
     AH_VARS_H2 = 0x17,  /* rho, rho*vx, rho*vy, E  : 2D hydrodynamics */
