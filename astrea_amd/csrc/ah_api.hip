@@ -242,7 +242,15 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     for (int c = 0, k = 0; c < 8; ++c)
         if ((sch->varmask >> c) & 1) p->varidx[k++] = c;
     bool ok;
-    if (sch->ppm_author != AH_PPM_MC)
+    // tile_s: the 1D sweep's tile length (0: the default); 115 selects the
+    // alternative shape of 1D hydrodynamic PLM/PPM with LLF/HLLC
+    if (sch->tile_s != 0 && (sch->dims != 1 || sch->ppm_author != AH_PPM_MC))
+        ok = false;
+    else if (sch->tile_s != 0)
+        ok = (sch->precision == AH_FP64 ? kernels_1d_small_fp64(sch->varmask, sch->recon, sch->solver, &p->ks)
+                                        : kernels_1d_small_fp32(sch->varmask, sch->recon, sch->solver, &p->ks)) &&
+             p->ks.ts1 == sch->tile_s;
+    else if (sch->ppm_author != AH_PPM_MC)
         ok = author_supported(sch) && kernels_1d_author_fp64(sch->ppm_author, sch->solver, &p->ks);
     else if (sch->precision == AH_FP64)
         ok = sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks)
@@ -1180,7 +1188,12 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
     if (n0 <= 0 || n1 <= 0 || q->dx <= 0 || q->checkpoints <= 0 || (q->boundary != AH_EDGE && q->boundary != AH_WRAP))
         return AH_ERR_INVALID;
     KernelSet ks;
-    bool ok = sch->ppm_author != AH_PPM_MC
+    if (sch->tile_s != 0 && (sch->dims != 1 || sch->ppm_author != AH_PPM_MC)) return AH_ERR_UNSUPPORTED;
+    bool ok = sch->tile_s != 0
+                  ? (sch->precision == AH_FP64 ? kernels_1d_small_fp64(sch->varmask, sch->recon, sch->solver, &ks)
+                                               : kernels_1d_small_fp32(sch->varmask, sch->recon, sch->solver, &ks)) &&
+                        ks.ts1 == sch->tile_s
+                  : sch->ppm_author != AH_PPM_MC
                   ? kernels_1d_author_fp64(sch->ppm_author, sch->solver, &ks)
                   : sch->precision == AH_FP64
                   ? (sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &ks)

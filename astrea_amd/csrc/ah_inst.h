@@ -5,14 +5,14 @@
 
 namespace ah {
 
-template <class R, int MASK, int RECON, int SOLVER, bool SPLIT, int AUTH = AUTH_MC>
+template <class R, int MASK, int RECON, int SOLVER, bool SPLIT, int AUTH = AUTH_MC, int NTHR1 = 256>
 inline void set1d(KernelSet* ks) {
     // cells per tile: the widest per-stage window (the q window, TS + 13)
-    // fits one pass of the 256 threads, so no stage runs a second, nearly
-    // empty round while the other waves wait at the barrier
-    constexpr int TS = VS<MASK>::NV > 4 ? 128 : 256 - 13;
+    // fits one pass of the workgroup's threads, so no stage runs a second,
+    // nearly empty round while the other waves wait at the barrier
+    constexpr int TS = VS<MASK>::NV > 4 ? 128 : NTHR1 - 13;
     ks->sweep1d = [](const SweepArgs& a, int nb, hipStream_t s) {
-        launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS, SPLIT, AUTH>, a, nb, s);
+        launch_sweep(sweep1d_kernel<R, MASK, RECON, SOLVER, TS, SPLIT, AUTH>, a, nb, s, NTHR1);
     };
     ks->ts1 = TS;
     ks->real_bytes = sizeof(R);
@@ -140,6 +140,19 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
         }
     }
     return false;
+}
+
+// 1D hydrodynamic PLM/PPM with LLF or HLLC on 115-cell tiles of 128-thread
+// workgroups (two waves): the alternative tile shape the ensemble autotuner
+// times against the 243-cell default
+template <class R, bool SPLIT>
+inline bool select_kernels_1d_small(int recon, int solver, KernelSet* ks) {
+    if (recon == PLM && solver == LLF) set1d<R, 0x13, PLM, LLF, SPLIT, AUTH_MC, 128>(ks);
+    else if (recon == PLM && solver == HLLC) set1d<R, 0x13, PLM, HLLC, SPLIT, AUTH_MC, 128>(ks);
+    else if (recon == PPM && solver == LLF) set1d<R, 0x13, PPM, LLF, SPLIT, AUTH_MC, 128>(ks);
+    else if (recon == PPM && solver == HLLC) set1d<R, 0x13, PPM, HLLC, SPLIT, AUTH_MC, 128>(ks);
+    else return false;
+    return true;
 }
 
 }  // namespace ah

@@ -59,12 +59,13 @@ def _start(plan, inits):
 
 
 def run_ensemble_batched(grids, probs, precision="fp64", chunk=32, use_graph=True, blocks=None, compat="physical",
-                         compact_every=4):
+                         compact_every=4, tile_s=0):
     """The ensemble in one plan, replayed in graph chunks of `chunk` steps with
     the tile lists compacted every `compact_every` chunks (on the 512-problem
     ensemble, 32 and 4 run 7 % faster than 16 and 1: each compaction waits for
     the device, while a finished problem's tiles only cost an early exit)."""
-    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat,
+                tile_s=tile_s)
     if blocks:
         plan.set_blocks(blocks)
     _start(plan, grids)
@@ -99,12 +100,30 @@ def run_ensemble_plain(grids, probs, sample=None, chunk=64, compat="physical"):
     return dict(seconds=secs, zu=zu, zu_per_s=zu / secs, status=st)
 
 
-def autotune_blocks(grids, probs, precision, candidates=(1024, 2048, 4096, 8192), steps=64, compat="physical"):
+def autotune(grids, probs, precision, tiles=(0, 115), candidates=(1024, 2048, 4096, 8192), steps=64,
+             compat="physical"):
+    """Pick the 1D tile shape (``tile_s``: 0 = 243-cell tiles of 256 threads,
+    115 = 115-cell tiles of 128 threads) and the persistent grid cap on the
+    problem set itself: time a fixed number of steps of the set for every
+    pair (each from the initial state).  Returns ((tile_s, blocks), times)."""
+    best, times = None, {}
+    for t in tiles:
+        b, tb = autotune_blocks(grids, probs, precision, candidates, steps, compat, tile_s=t)
+        for k, v in tb.items():
+            times[(t, k)] = v
+            if best is None or v < times[best]:
+                best = (t, k)
+    return best, times
+
+
+def autotune_blocks(grids, probs, precision, candidates=(1024, 2048, 4096, 8192), steps=64, compat="physical",
+                    tile_s=0):
     """Pick the persistent grid cap on the problem set itself: time a
     fixed number of steps for each candidate (state restored between)."""
     best, times = None, {}
     for b in candidates:
-        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat)
+        plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, precision=precision, compat=compat,
+                    tile_s=tile_s)
         plan.set_blocks(b)
         _start(plan, grids)
         plan.reset()

@@ -266,8 +266,9 @@ def ensemble_leg(args):
     import torch
     from astrea_amd import workloads as W
     grids, probs = W.ensemble_inputs(args.nprob, SEED)
-    blocks, tune = W.autotune_blocks(grids, probs, "fp64")
-    res = {p: W.run_ensemble_batched(grids, probs, precision=p, blocks=blocks) for p in ("fp64", "split")}
+    (tile_s, blocks), tune = W.autotune(grids, probs, "fp64")
+    res = {p: W.run_ensemble_batched(grids, probs, precision=p, blocks=blocks, tile_s=tile_s)
+           for p in ("fp64", "split")}
     _progress(f"[ensemble] batched fp64 {res['fp64']['zu_per_s']:.3e} ZU/s, split {res['split']['zu_per_s']:.3e}")
     host, _ = W.ensemble_inputs(args.nprob, SEED, host_grids=True)
     gen = torch.Generator(device=host[0].device).manual_seed(SEED)
@@ -276,7 +277,7 @@ def ensemble_leg(args):
         sign = 2 * torch.randint(0, 2, g.shape, generator=gen, device=g.device, dtype=torch.float64) - 1
         pert.append(g * (1 + 1e-15 * sign))
     assert all(not torch.equal(a, b) for a, b in zip(pert, host))
-    res_p = W.run_ensemble_batched(pert, probs, precision="fp64", blocks=blocks)
+    res_p = W.run_ensemble_batched(pert, probs, precision="fp64", blocks=blocks, tile_s=tile_s)
     del host, pert
     ref = res["fp64"]
     split_err, sens, mism = {}, {}, 0
@@ -314,7 +315,8 @@ def ensemble_leg(args):
         "fp64_l1_under_1e-15_perturbation": stat(list(sens.values())),
         "worst_split_problems": [{"problem": i, "n": probs[i]["n0"], "split_l1": split_err[i],
                                   "fp64_perturbed_l1": sens.get(i)} for i in worst],
-        "autotune_blocks": {"picked": blocks, "seconds": {str(k): v for k, v in tune.items()}},
+        "autotune": {"picked": {"tile_s": tile_s or 243, "blocks": blocks},
+                     "seconds": {f"tile_s={t or 243},blocks={b}": v for (t, b), v in tune.items()}},
     }
 
 
@@ -373,16 +375,17 @@ def bench_ensemble(args):
     import torch
     from astrea_amd import workloads as W
     grids, probs = W.ensemble_inputs(args.nprob, SEED)
-    blocks, tune = W.autotune_blocks(grids, probs, "fp64")
+    (tile_s, blocks), tune = W.autotune(grids, probs, "fp64")
     res = {}
     paths = args.ens_paths.split(",")
     for prec in ("fp64", "split"):
         if prec in paths or prec == "fp64":
-            res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, blocks=blocks)
+            res[prec] = W.run_ensemble_batched(grids, probs, precision=prec, blocks=blocks, tile_s=tile_s)
             _progress(f"[ensemble] batched {prec}: {res[prec]['zu_per_s']:.3e} ZU/s")
     literal = None
     if "literal" in paths:
-        literal = W.run_ensemble_batched(grids, probs, precision="fp64", blocks=blocks, compat="astrea")
+        literal = W.run_ensemble_batched(grids, probs, precision="fp64", blocks=blocks, compat="astrea",
+                                         tile_s=tile_s)
     ref = res["fp64"]
     errs, mism = [], 0
     for i in range(len(probs)):
@@ -419,7 +422,8 @@ def bench_ensemble(args):
         "split_l1_vs_fp64": {"max": max(errs) if errs else None, "median": float(np.median(errs)) if errs else None,
                              "p90": float(np.percentile(errs, 90)) if errs else None, "compared": len(errs),
                              "failure_mismatch": mism},
-        "autotune_blocks": {"picked": blocks, "seconds": {str(k): v for k, v in tune.items()}},
+        "autotune": {"picked": {"tile_s": tile_s or 243, "blocks": blocks},
+                     "seconds": {f"tile_s={t or 243},blocks={b}": v for (t, b), v in tune.items()}},
     }
 
 

@@ -100,8 +100,9 @@ typedef struct ah_scheme {
     int precision;   /* ah_precision */
     int varmask;     /* ah_varmask */
     int magnetic_2d; /* constrained transport (evolvers.py:26-32, 51-58, 63-67, 73-76) */
-    int tile_s;      /* sweep tile length in cells (0: default / autotuned) */
-    int tile_t;      /* transverse tile width in lines, 2D only (0: default) */
+    int tile_s;      /* 1D sweep tile length in cells: 0 (the default, 243 for hydrodynamics) or
+                        115 (128-thread workgroups; 1D hydrodynamic PLM/PPM with LLF/HLLC) */
+    int tile_t;      /* reserved (0) */
     int compat;      /* ah_compat */
     int av;          /* 1: artificial viscosity on the fluxes (1D PPM only; schemes/ppm.py:138-170,
                         whose output the reference computes but never applies: 0 is the

@@ -434,3 +434,43 @@ def test_ppm_author_refused_outside_1d_hydro():
         Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x17, ppm_author="c")
     with pytest.raises(L.AstreaHipError, match="not implemented"):
         Plan(1, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x13, ppm_author="ph", flattener="astrea")
+
+
+# ---- tile shapes of the 1D sweep (the autotuner's choices) -----------------------
+@pytest.mark.parametrize("subgrid,solver,precision", [("ppm", "hllc", "fp64"), ("plm", "llf", "fp64"),
+                                                      ("ppm", "hllc", "split")])
+def test_tile_shapes_give_the_same_bits(subgrid, solver, precision):
+    """A ragged batch on the 115-cell tiles (128-thread workgroups) ends in
+    the same state, step counts and failure flags as on the default 243-cell
+    tiles, bit for bit: the tile shape changes the schedule, not the
+    arithmetic."""
+    import torch
+    from astrea_amd import batch, ic
+    from astrea_amd.plan import Plan
+    specs = ic.ensemble_1d(9, 3, 40, 700)
+    probs = [dict(n0=sp["n"], boundary="edge", dx=1 / sp["n"], gamma=1.4, cfl=.5, t_end=sp["t_end"] / 8)
+             for sp in specs]
+    outs = []
+    for tile_s in (0, 115):
+        _say(f"[gpu] tile shape {tile_s or 243}: {subgrid}/{solver}/{precision}, sizes {[p['n0'] for p in probs]}")
+        plan = Plan(1, subgrid, solver, "ssprk(3,3)", probs, varmask=0x13, precision=precision, tile_s=tile_s)
+        for i, sp in enumerate(specs):
+            plan.init_riemann(i, sp["left"], sp["right"])
+        plan.reset()
+        st, _, _ = batch.run(plan, chunk=8)
+        outs.append((plan.state.clone(), [(s["steps"], s["failed"]) for s in st]))
+        plan.close()
+    assert outs[0][1] == outs[1][1]
+    assert torch.equal(outs[0][0], outs[1][0])
+
+
+def test_tile_shape_refused_where_not_built():
+    from astrea_amd import _lib as L
+    from astrea_amd.plan import Plan
+    prob = dict(n0=64, n1=64, boundary="wrap", dx=1 / 64, gamma=1.4, cfl=.4, t_end=1.)
+    for kw in (dict(dims=2, subgrid="ppm", varmask=0x17, tile_s=115), dict(dims=1, subgrid="weno5", varmask=0x13,
+                                                                           tile_s=115),
+               dict(dims=1, subgrid="ppm", varmask=0x13, tile_s=100)):
+        dims, sg = kw.pop("dims"), kw.pop("subgrid")
+        with pytest.raises(L.AstreaHipError, match="not implemented"):
+            Plan(dims, sg, "hllc", "ssprk(3,3)", [prob], **kw)
