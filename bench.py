@@ -564,6 +564,17 @@ def bench_mhd(args):
     b_alg_zu = 8 * 6 * 8
     per_launch = b_alg_zu * n * n / 6
     achieved = per_launch / (dom_ms / 1e3) / 1e9
+    # HBM bytes and VALU instructions of that launch from the counter runs
+    # summarised in profiles/ (2048^2 only)
+    traffic, valu, tsrc = None, None, None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_ot2048.json")
+    if n == 2048 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic, tsrc = tj.get("bytes_per_launch"), tj.get("source")
+        issue_s = tj["valu_wave_insts_per_launch"] * 4 / (1024 * 2.4e9)
+        valu = {"wave_insts_per_launch": tj["valu_wave_insts_per_launch"], "issue_ms": issue_s * 1e3,
+                "frac": issue_s / (dom_ms / 1e3), "source": tsrc}
     cpu = None if args.no_cpu else cpu_baseline(128, 8, "mhd")
     return {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -572,8 +583,10 @@ def bench_mhd(args):
         "config": {"workload": f"2D MHD Orszag-Tang {n}x{n} periodic, WENO5/HLLD/constrained transport/SSPRK(3,3), "
                                "CFL 0.4, gamma 5/3, solver axis = physical sweep axis", "cells": n * n},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": dom, "kernel_avg_ms": dom_ms,
-                     "bytes_per_launch": per_launch},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "kernel": dom, "kernel_avg_ms": dom_ms,
+                     "bytes_per_launch": per_launch,
+                     "traffic_source": None if traffic is None else
+                     f"rocprofv3 FETCH_SIZE/WRITE_SIZE of {dom} at 2048^2, calibrated ({tsrc})", "valu": valu},
         "cpu_baseline": cpu,
         "kernels_avg_ms": {k: v[0] / v[1] for k, v in prof.items()},
         "div_b": {"max_initial": div0[0], "max_final": div1[0], "max_final_over_Bmax_per_dx": div1[1],
