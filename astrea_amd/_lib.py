@@ -40,7 +40,8 @@ class AhScheme(ctypes.Structure):
                 ("flattener", ctypes.c_int), ("solver", ctypes.c_int), ("rk", ctypes.c_int),
                 ("precision", ctypes.c_int), ("varmask", ctypes.c_int), ("magnetic_2d", ctypes.c_int),
                 ("tile_s", ctypes.c_int), ("tile_t", ctypes.c_int), ("compat", ctypes.c_int),
-                ("av", ctypes.c_int), ("ppm_author", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
+                ("av", ctypes.c_int), ("ppm_author", ctypes.c_int), ("fuse2d", ctypes.c_int),
+                ("reserved", ctypes.c_int * 1)]
 
 
 class AhProblem(ctypes.Structure):

@@ -59,6 +59,13 @@ std::vector<Stage> rk_program(int rk) {
 // PPM authors c/ph (ppm.py:43-62): 1D hydrodynamics (whose zero components
 // make the extrapolant limiter's grid-wide test always true), FP64, without
 // the flattener and artificial viscosity the reference ties to author 'mc'
+// the one-launch 2D sweep: hydrodynamic PLM/PPM with LLF or HLLC
+bool fused_supported(const ah_scheme* sch) {
+    return sch->dims == 2 && sch->varmask == AH_VARS_H2 && !sch->magnetic_2d && sch->tile_s == 0 &&
+           sch->ppm_author == AH_PPM_MC && !sch->av && (sch->recon == AH_PLM || sch->recon == AH_PPM) &&
+           (sch->solver == AH_LLF || sch->solver == AH_HLLC);
+}
+
 bool author_supported(const ah_scheme* sch) {
     return (sch->ppm_author == AH_PPM_C || sch->ppm_author == AH_PPM_PH) && sch->dims == 1 &&
            sch->recon == AH_PPM && sch->varmask == AH_VARS_H1 && sch->precision == AH_FP64 &&
@@ -246,6 +253,10 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     // alternative shape of 1D hydrodynamic PLM/PPM with LLF/HLLC
     if (sch->tile_s != 0 && (sch->dims != 1 || sch->ppm_author != AH_PPM_MC))
         ok = false;
+    else if (sch->fuse2d)
+        ok = fused_supported(sch) &&
+             (sch->precision == AH_FP64 ? kernels_2d_fused_fp64(sch->recon, sch->solver, &p->ks)
+                                        : kernels_2d_fused_fp32(sch->recon, sch->solver, &p->ks));
     else if (sch->tile_s != 0)
         ok = (sch->precision == AH_FP64 ? kernels_1d_small_fp64(sch->varmask, sch->recon, sch->solver, &p->ks)
                                         : kernels_1d_small_fp32(sch->varmask, sch->recon, sch->solver, &p->ks)) &&
@@ -304,8 +315,9 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
         } else {
             for (int ax = 0; ax < 2; ++ax) {
                 const int ns = ax == 0 ? q.n0 : n1, nt = ax == 0 ? n1 : q.n0;
-                for (int t0 = 0; t0 < nt; t0 += p->ks.rtt[ax])
-                    for (int s0 = 0; s0 < ns; s0 += p->ks.rts[ax]) p->tl[1 + ax].full.push_back({i, s0, t0, 0});
+                if (p->ks.recon[ax])  // no reconstruction launch when the flux tiles reconstruct
+                    for (int t0 = 0; t0 < nt; t0 += p->ks.rtt[ax])
+                        for (int s0 = 0; s0 < ns; s0 += p->ks.rts[ax]) p->tl[1 + ax].full.push_back({i, s0, t0, 0});
                 for (int t0 = 0; t0 < nt; t0 += p->ks.ftt[ax])
                     for (int s0 = 0; s0 < ns; s0 += p->ks.fts[ax]) p->tl[3 + ax].full.push_back({i, s0, t0, 0});
                 if (sch->magnetic_2d)
@@ -391,7 +403,7 @@ int ah_plan_workspace_size(const ah_plan* p, size_t* bytes) {
     size_t b = plane_bytes(p, 8) * (p->nregs + 1);  // registers + L
     if (p->sch.dims == 2) {
         b += plane_bytes(p, 8);                        // D
-        b += 2 * plane_bytes(p, p->ks.real_bytes);     // wL, wR
+        if (p->ks.recon[0]) b += 2 * plane_bytes(p, p->ks.real_bytes);  // wL, wR
         b += ((size_t)p->total * p->ks.real_bytes + 255) & ~size_t(255);  // normal field for HLLD
         if (p->sch.magnetic_2d) {
             b += 6 * plane_bytes(p, p->ks.real_bytes);                          // wF, wD, wU per sweep
@@ -441,10 +453,13 @@ int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
     if (p->sch.dims == 2) {
         p->Dbuf = (double*)c;
         c += plane_bytes(p, 8);
-        p->wL = c;
-        c += plane_bytes(p, p->ks.real_bytes);
-        p->wR = c;
-        c += plane_bytes(p, p->ks.real_bytes);
+        p->wL = p->wR = nullptr;
+        if (p->ks.recon[0]) {
+            p->wL = c;
+            c += plane_bytes(p, p->ks.real_bytes);
+            p->wR = c;
+            c += plane_bytes(p, p->ks.real_bytes);
+        }
         p->bn = c;
         c += ((size_t)p->total * p->ks.real_bytes + 255) & ~size_t(255);
         if (p->sch.magnetic_2d) {
@@ -536,7 +551,7 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.tiles = p->tl[1 + ax].d;
         a.ntiles_dev = p->tl[1 + ax].d_count;
         a.wF = p->sch.magnetic_2d ? p->wF[ax] : nullptr;
-        {
+        if (p->ks.recon[ax]) {
             ProfScope ps_(p, ax == 0 ? AH_K_RECON0 : AH_K_RECON1, s);
             p->ks.recon[ax](a, blocks_for(p, p->tl[1 + ax]), s);
         }
@@ -1189,7 +1204,11 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
         return AH_ERR_INVALID;
     KernelSet ks;
     if (sch->tile_s != 0 && (sch->dims != 1 || sch->ppm_author != AH_PPM_MC)) return AH_ERR_UNSUPPORTED;
-    bool ok = sch->tile_s != 0
+    if (sch->fuse2d && !fused_supported(sch)) return AH_ERR_UNSUPPORTED;
+    bool ok = sch->fuse2d
+                  ? (sch->precision == AH_FP64 ? kernels_2d_fused_fp64(sch->recon, sch->solver, &ks)
+                                               : kernels_2d_fused_fp32(sch->recon, sch->solver, &ks))
+                  : sch->tile_s != 0
                   ? (sch->precision == AH_FP64 ? kernels_1d_small_fp64(sch->varmask, sch->recon, sch->solver, &ks)
                                                : kernels_1d_small_fp32(sch->varmask, sch->recon, sch->solver, &ks)) &&
                         ks.ts1 == sch->tile_s
@@ -1208,7 +1227,9 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
     TileSpec spec[4];
     int nlists = 0;
     auto add = [&](int ns, int nt, int ts, int tt) {
-        const int n = ((ns + ts - 1) / ts) * ((nt + tt - 1) / tt);
+        // ts = 0: a list the scheme does not launch (the one-launch 2D sweep
+        // has no reconstruction tiles)
+        const int n = ts ? ((ns + ts - 1) / ts) * ((nt + tt - 1) / tt) : 0;
         spec[nlists++] = TileSpec{nullptr, nullptr, ns, ts, tt, n};
     };
     if (sch->dims == 1) {
@@ -1224,7 +1245,7 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
     size_t need = align256(sizeof(ProbDesc)) + align256(sizeof(ProbState)) + align256(4 * sizeof(int));
     for (int l = 0; l < nlists; ++l) need += align256((size_t)spec[l].n * sizeof(Tile));
     need += plane8;  // L
-    if (sch->dims == 2) need += plane8 + 2 * planeR + align256((size_t)cells * ks.real_bytes);
+    if (sch->dims == 2) need += plane8 + (ks.recon[0] ? 2 * planeR : 0) + align256((size_t)cells * ks.real_bytes);
     if (need > (size_t)AH_STATIC_SCRATCH_BYTES || cells >= (1ll << 31)) return AH_ERR_TOO_LARGE;
     unsigned char* base = nullptr;
     int rc = scratch_ptr(&base);
@@ -1248,10 +1269,12 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
     if (sch->dims == 2) {
         Dbuf = (double*)c;
         c += plane8;
-        wL = c;
-        c += planeR;
-        wR = c;
-        c += planeR;
+        if (ks.recon[0]) {
+            wL = c;
+            c += planeR;
+            wR = c;
+            c += planeR;
+        }
         bn = c;
     }
     ProbDesc desc;
@@ -1311,7 +1334,7 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
         for (int ax = 0; ax < 2; ++ax) {
             a.tiles = spec[ax].dst;
             a.ntiles_dev = spec[ax].count;
-            ks.recon[ax](a, blocks(ax), s);
+            if (ks.recon[ax]) ks.recon[ax](a, blocks(ax), s);
             a.tiles = spec[2 + ax].dst;
             a.ntiles_dev = spec[2 + ax].count;
             a.last = ax == 1;

@@ -142,6 +142,38 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
     return false;
 }
 
+// 2D hydrodynamics, one launch per sweep: the flux kernel reconstructs its
+// own window (recon[] stays null, and the plan keeps no wL/wR planes).  The
+// 16 x 8 tile takes 52 KB of LDS (the reconstruction windows share theirs
+// with the face-average arrays): three workgroups per CU.
+template <class R, int RECON, int SOLVER, bool SPLIT>
+inline void set2d_fused(KernelSet* ks) {
+    constexpr int CONV = RECON == PLM ? 1 : 2;
+    constexpr int TS = 16, TT = 8, NT = 256;
+    ks->flux[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
+        launch_sweep(flux2d_kernel<R, 0x17, SOLVER, CONV, 0, TS, TT, SPLIT, NT, RECON>, a, nb, s, NT);
+    };
+    ks->flux[1] = [](const SweepArgs& a, int nb, hipStream_t s) {
+        launch_sweep(flux2d_kernel<R, 0x17, SOLVER, CONV, 1, TS, TT, SPLIT, NT, RECON>, a, nb, s, NT);
+    };
+    for (int ax = 0; ax < 2; ++ax) {
+        ks->recon[ax] = nullptr;
+        ks->fts[ax] = TS;
+        ks->ftt[ax] = TT;
+    }
+    ks->real_bytes = sizeof(R);
+}
+
+template <class R, bool SPLIT>
+inline bool select_kernels_2d_fused(int recon, int solver, KernelSet* ks) {
+    if (recon == PLM && solver == LLF) set2d_fused<R, PLM, LLF, SPLIT>(ks);
+    else if (recon == PLM && solver == HLLC) set2d_fused<R, PLM, HLLC, SPLIT>(ks);
+    else if (recon == PPM && solver == LLF) set2d_fused<R, PPM, LLF, SPLIT>(ks);
+    else if (recon == PPM && solver == HLLC) set2d_fused<R, PPM, HLLC, SPLIT>(ks);
+    else return false;
+    return true;
+}
+
 // 1D hydrodynamic PLM/PPM with LLF or HLLC on 115-cell tiles of 128-thread
 // workgroups (two waves): the alternative tile shape the ensemble autotuner
 // times against the 243-cell default

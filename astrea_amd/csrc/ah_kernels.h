@@ -1007,22 +1007,63 @@ void recon2d_kernel(SweepArgs a) {
 // transverse correction (functions/fv.py:147-153) -> flux difference.
 // CONV: 0 = PCM (q from the cell state), 1 = point (PLM), 2 = 4th-order face
 // conversion (PPM/WENO, fv.py:105-122 with 'face').
+// FRECON >= 0 (hydrodynamics): one launch per sweep.  The tile reconstructs
+// (scheme FRECON) the interface states of its own window, the cells
+// [s0-1, s0+TS+1] on the lines [t0-2, t0+TT+2), straight into LDS instead of
+// reading the wL/wR planes a reconstruction launch wrote: the same cell
+// evaluation as recon2d_kernel on the same boundary-mapped window, so the
+// window holds bit for bit the states that launch writes.
 // ============================================================================
-template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT, bool SPLIT = false, int NTHR = 256>
-__global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
+template <class R, int MASK, int SOLVER, int CONV, int AX, int TS, int TT, bool SPLIT = false, int NTHR = 256,
+          int FRECON = -1>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(FRECON >= 0 ? 3 : 1)))
+void flux2d_kernel(SweepArgs a) {
     using V = VS<MASK>;
     constexpr int NV = V::NV;
     constexpr int NK = TS + 2;   // interfaces [s0, s0+TS+1]
     constexpr int NT = TT + 4;   // w lines   [t0-2, t0+TT+2)
     constexpr int NKF = TS + 1;  // flux interfaces [s0, s0+TS]
     constexpr int NTA = TT + 2;  // face-average lines [t0-1, t0+TT+1)
+    constexpr bool FUSED = FRECON >= 0;
+    static_assert(!FUSED || !V::MHD, "the in-tile reconstruction writes no constrained-transport face values");
     __shared__ R wm_s[NV][NT][NK];
     __shared__ R wp_s[NV][NT][NK];
-    __shared__ R qm_s[NV][NTA][NKF];
-    __shared__ R qp_s[NV][NTA][NKF];
-    __shared__ R fm_s[NV][NTA][NKF];
-    __shared__ R fp_s[NV][NTA][NKF];
-    __shared__ R fa_s[NV][NTA][NKF];
+    // In-tile reconstruction windows (FUSED): q and its primitive form on the
+    // lines [t0-2-FHT, t0+TT+2+FHT) and positions [s0-1-FHS-1, s0+TS+2+FHS],
+    // wS on the lines [t0-2, t0+TT+2) and positions [s0-1-FHS, s0+TS+2+FHS),
+    // the flattener's chi~ at [s0-3, s0+TS+3).  They are dead once the
+    // interface states are in wm_s/wp_s and share their LDS with the
+    // face-average arrays of the solve stages.
+    constexpr bool FHO = FRECON == PPM || FRECON >= WENO3;
+    constexpr int FHS = FHO ? 4 : 1;
+    constexpr int FNC = TS + 3;
+    constexpr int FNWS = FNC + 2 * FHS;
+    constexpr int FNQS = FNWS + 2;
+    constexpr int FHT = FHO ? 1 : 0;
+    constexpr int FNQT = NT + 2 * FHT;
+    constexpr int FNCHI = TS + 6;
+    constexpr long long FQ_B = (long long)NV * FNQT * FNQS * 8;
+    constexpr long long FREC_B = FQ_B * (FHO ? 2 : 1) + (long long)NV * NT * FNWS * sizeof(R) +
+                                 (FRECON == PPM ? (long long)NT * FNCHI * sizeof(R) : 0);
+    constexpr long long FFACE_B = 5ll * NV * NTA * NKF * sizeof(R);
+    constexpr long long FBUF_B = FUSED ? (FREC_B > FFACE_B ? FREC_B : FFACE_B) : 8;
+    __shared__ double fbuf[(FBUF_B + 7) / 8];
+    constexpr int FA = FUSED ? 1 : NV, FB = FUSED ? 1 : NTA, FC = FUSED ? 1 : NKF;
+    __shared__ R qm_a[FA][FB][FC];
+    __shared__ R qp_a[FA][FB][FC];
+    __shared__ R fm_a[FA][FB][FC];
+    __shared__ R fp_a[FA][FB][FC];
+    __shared__ R fa_a[FA][FB][FC];
+    using FaceArr = R (*)[NTA][NKF];
+    auto face_arr = [&](R* own, int k) -> FaceArr {
+        if constexpr (FUSED)
+            return reinterpret_cast<FaceArr>(reinterpret_cast<char*>(fbuf) + k * (long long)NV * NTA * NKF * sizeof(R));
+        else
+            return reinterpret_cast<FaceArr>(own);
+    };
+    const FaceArr qm_s = face_arr(&qm_a[0][0][0], 0), qp_s = face_arr(&qp_a[0][0][0], 1),
+                  fm_s = face_arr(&fm_a[0][0][0], 2), fp_s = face_arr(&fp_a[0][0][0], 3),
+                  fa_s = face_arr(&fa_a[0][0][0], 4);
     // only LLF and Lax-Wendroff read coefficients of the halo positions (the
     // radii, or ss^2 / radius); the other solvers need the radius of the
     // tile's own interfaces (dt reduction and the non-finite check), which
@@ -1047,9 +1088,10 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     // fetched into registers one tile ahead: the loads of the next tile are in
     // flight while this tile computes (residency is bounded by the LDS windows
     // before the registers, so the prefetch costs no occupancy)
-    constexpr int NA = (NT * NK + NTHR - 1) / NTHR;
+    constexpr int NA = FUSED ? 1 : (NT * NK + NTHR - 1) / NTHR;
     R pfm[NA][NV], pfp[NA][NV];
     auto fetch = [&](int ti) {
+        if constexpr (FUSED) return;
         const Tile tl = ld_const(a.tiles + ti);
         const ProbDesc q = ld_const(a.pd + tl.prob);
         const int ns_ = AX == 0 ? q.n0 : q.n1, nt_ = AX == 0 ? q.n1 : q.n0;
@@ -1076,7 +1118,7 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     ProbState* ps = a.ps + tile.prob;
     const bool active = ld_const(&ps->active);
     // A: the prefetched window into LDS, then start the next tile's loads
-    if (active) {
+    if (!FUSED && active) {
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
             const int i = threadIdx.x + j * NTHR;
@@ -1103,6 +1145,112 @@ __global__ __launch_bounds__(NTHR) void flux2d_kernel(SweepArgs a) {
     const int bt = t0 - 2, bta = t0 - 1;
     auto sidx = [&](int s, int t) { return off + (long long)t * ns + s; };  // sweep-frame layout
     auto gidx = [&](int s, int t) { return off + (AX == 0 ? (long long)s * pd.n1 + t : (long long)t * pd.n1 + s); };
+    if constexpr (FUSED) {
+        // R-A: q (and its primitive form) of the window, boundary mapped
+        using QArr = double (*)[FNQT][FNQS];
+        using WArr = R (*)[NT][FNWS];
+        const QArr qd = reinterpret_cast<QArr>(fbuf);
+        const QArr cq = reinterpret_cast<QArr>(reinterpret_cast<char*>(fbuf) + FQ_B);
+        const WArr ws = reinterpret_cast<WArr>(reinterpret_cast<char*>(fbuf) + FQ_B * (FHO ? 2 : 1));
+        R (*chi_s)[FNCHI] =
+            reinterpret_cast<R (*)[FNCHI]>(reinterpret_cast<char*>(ws) + (long long)NV * NT * FNWS * sizeof(R));
+        const double gamma = pd.gamma;
+        const int bqs = s0 - 1 - FHS - 1, bqt = t0 - 2 - FHT, bws = s0 - 1 - FHS;
+        auto es = [&](int p) { return bc == EDGE ? clampi(p, 0, ns - 1) : p; };
+        auto etl = [&](int p) { return bc == EDGE ? clampi(p, 0, nt - 1) : p; };
+        int rhit_;
+        const double rxdt = a.xl ? step_dt(pd, *ps, 2, &rhit_) : 0.0;
+        for (int i = threadIdx.x; i < FNQT * FNQS; i += blockDim.x) {
+            int is, it;
+            if constexpr (AX == 0) { is = i / FNQT; it = i % FNQT; }
+            else { it = i / FNQS; is = i % FNQS; }
+            const int gs = bcmap(bqs + is, ns, bc), gt = bcmap(bqt + it, nt, bc);
+            const long long g = gidx(gs, gt);
+            V8<double> q = zero8<double, MASK>();
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) q.c[c] = stage_x(a, V::slot(c) * P + g, rxdt);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) qd[V::slot(c)][it][is] = q.c[c];
+            if constexpr (FHO) {
+                const V8<double> w = c2p<MASK, SPLIT>(q, gamma);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) cq[V::slot(c)][it][is] = w.c[c];
+            }
+        }
+        __syncthreads();
+        // R-B: wS on every line of the window (line t0-2+ir evaluated on the
+        // boundary-mapped line, as the reconstruction launch evaluates its own)
+        for (int i = threadIdx.x; i < NT * FNWS; i += blockDim.x) {
+            const int ir = i / FNWS, is = i % FNWS;
+            const int js = es(bws + is) - bqs;
+            const int jt = etl(t0 - 2 + ir) - bqt;
+            auto Q = [&](int x, int y) {
+                V8<double> v = zero8<double, MASK>();
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) v.c[c] = qd[V::slot(c)][y][x];
+                return v;
+            };
+            V8<double> w;
+            if constexpr (FHO) {
+                auto CQ = [&](int x, int y) {
+                    V8<double> v = zero8<double, MASK>();
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                        if (V::live(c)) v.c[c] = cq[V::slot(c)][y][x];
+                    return v;
+                };
+                w = ho_cell_prim<MASK, 2, SPLIT>(Q(js, jt), Q(js - 1, jt), Q(js + 1, jt), Q(js, jt - 1),
+                                                 Q(js, jt + 1), CQ(js, jt), CQ(js - 1, jt), CQ(js + 1, jt),
+                                                 CQ(js, jt - 1), CQ(js, jt + 1), gamma);
+            } else {
+                w = c2p<MASK, SPLIT>(Q(js, jt), gamma);
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) ws[V::slot(c)][ir][is] = R(w.c[c]);
+        }
+        __syncthreads();
+        // R-C: cells [s0-1, s0+TS+1] of every line: cell s0-1+j gives w- of
+        // interface s0+j (its wR) and w+ of interface s0+j-1 (its wL)
+        const bool clampS = bc == EDGE && !(s0 - 1 - FHS >= 0 && s0 + TS + 1 + FHS < ns);
+        auto rcells = [&](auto clamp_tag) {
+            constexpr bool CL = decltype(clamp_tag)::value;
+            if constexpr (FRECON == PPM) {
+                if (a.flattener != FLAT_OFF) {
+                    for (int i = threadIdx.x; i < NT * FNCHI; i += blockDim.x) {
+                        const int ir = i / FNCHI, j = i % FNCHI;
+                        const RowAcc<R, MASK, NT, FNWS, CL> W{ws, ir, bws, ns};
+                        chi_s[ir][j] = flat_chi_at<R, AX>(W, a.flattener, s0 - 3 + j);
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = threadIdx.x; i < NT * FNC; i += blockDim.x) {
+                const int ir = i / FNC, j = i % FNC;
+                const int v = s0 - 1 + j;
+                const int ce = CL ? clampi(v, 0, ns - 1) : v;
+                const RowAcc<R, MASK, NT, FNWS, CL> W{ws, ir, bws, ns};
+                V8<R> L, Rr;
+                if constexpr (FRECON == PPM)
+                    recon_cell<R, MASK, FRECON, AX>(W, ce, a.limiter, a.flattener, L, Rr,
+                                                    [&](int mm) { return chi_s[ir][mm - (s0 - 3)]; });
+                else
+                    recon_cell<R, MASK, FRECON, AX>(W, ce, a.limiter, a.flattener, L, Rr);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (V::live(c)) {
+                        if (j < NK) wm_s[V::slot(c)][ir][j] = Rr.c[c];
+                        if (j >= 1) wp_s[V::slot(c)][ir][j - 1] = L.c[c];
+                    }
+            }
+        };
+        if (clampS) rcells(std::true_type{});
+        else rcells(std::false_type{});
+    }
     __syncthreads();
     auto WM = [&](int it, int ik) {
         V8<R> w = zero8<R, MASK>();

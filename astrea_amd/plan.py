@@ -35,7 +35,7 @@ def infer_varmask(grids, dims, magnetic):
 class Plan:
     def __init__(self, dims, subgrid, solver, timestep, problems, *, limiter="minmod", flattener="off",
                  precision="fp64", varmask=L.AH_VARS_H1, compat="astrea", magnetic_2d=False, device="cuda:0",
-                 solver_category=None, av=False, ppm_author="mc", tile_s=0):
+                 solver_category=None, av=False, ppm_author="mc", tile_s=0, fuse2d=False):
         self.lib = L.load()
         self.dims = dims
         self.device = torch.device(device)
@@ -53,6 +53,7 @@ class Plan:
         sch.compat = L.AH_COMPAT_PHYSICAL if compat == "physical" else L.AH_COMPAT_ASTREA
         sch.av = 1 if av else 0
         sch.tile_s = int(tile_s)
+        sch.fuse2d = 1 if fuse2d else 0
         sch.ppm_author = {"mc": L.AH_PPM_MC, "c": L.AH_PPM_C, "ph": L.AH_PPM_PH}[ppm_author]
         self.scheme = sch
         arr = (L.AhProblem * len(problems))()

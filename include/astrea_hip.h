@@ -110,7 +110,11 @@ typedef struct ah_scheme {
     int ppm_author;  /* ah_ppm_author (ppm.py:13; the reference's evolve_space always passes 'mc',
                         evolvers.py:17); c / ph: 1D hydrodynamics in FP64 without flattener or
                         artificial viscosity */
-    int reserved[2];
+    int fuse2d;      /* 2D hydrodynamics (PLM/PPM with LLF/HLLC): 1 = one launch per sweep, the
+                        interface states reconstructed inside each flux tile instead of written to
+                        and read back from HBM by a reconstruction launch; the same results bit for
+                        bit, and no interface-state planes in the workspace.  0: two launches */
+    int reserved[1];
 } ah_scheme;
 
 typedef struct ah_problem {

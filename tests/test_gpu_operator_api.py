@@ -474,3 +474,90 @@ def test_tile_shape_refused_where_not_built():
         dims, sg = kw.pop("dims"), kw.pop("subgrid")
         with pytest.raises(L.AstreaHipError, match="not implemented"):
             Plan(dims, sg, "hllc", "ssprk(3,3)", [prob], **kw)
+
+
+# ---- one launch per 2D sweep (reconstruction inside the flux tile) --------------
+def _grid2d(n0, n1, seed):
+    """A sheared two-state layer with a pressure jump across x = 0.5 (the
+    flattener and the limiters' extremum branches both act), n0 x n1."""
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid((np.arange(n0) + .5) / n0, (np.arange(n1) + .5) / n1, indexing="ij")
+    w = np.zeros((n0, n1, 8))
+    w[..., 0] = np.where((X > .5) ^ (Y > .4), 1.5, .4) + .05 * rng.random((n0, n1))
+    w[..., 1] = np.where(Y > .4, .3, -.2) + .01 * rng.random((n0, n1))
+    w[..., 2] = .1 * np.sin(2 * np.pi * X)
+    w[..., 4] = np.where(X > .5, 1., .15) + .01 * rng.random((n0, n1))
+    return w
+
+
+FUSED_CASES = [
+    ("ppm", "hllc", "fp64", "astrea", "physical"),
+    ("plm", "hllc", "fp64", "off", "physical"),
+    ("ppm", "llf", "fp64", "off", "astrea"),
+    ("plm", "llf", "split", "off", "astrea"),
+    ("ppm", "hllc", "split", "astrea", "physical"),
+    ("ppm", "hllc", "fp64", "colella90", "astrea"),
+]
+
+
+@pytest.mark.parametrize("subgrid,solver,precision,flattener,compat", FUSED_CASES)
+def test_fused_sweep_gives_the_same_bits(subgrid, solver, precision, flattener, compat):
+    """A ragged 2D batch (tile-multiple and ragged sizes, grids smaller than
+    one tile, both boundaries) stepped with one launch per sweep ends in the
+    same state, step counts, times and failure flags as with the
+    reconstruction and flux launches, bit for bit."""
+    import torch
+    from astrea_amd.plan import Plan
+    shapes = [(64, 48, "wrap"), (37, 21, "edge"), (5, 9, "wrap"), (6, 70, "edge"), (33, 17, "wrap")]
+    probs = [dict(n0=a, n1=b, boundary=bc, dx=1 / max(a, b), gamma=1.4, cfl=.4, t_end=1., max_steps=9)
+             for a, b, bc in shapes]
+    grids = []
+    for k, (a, b, bc) in enumerate(shapes):
+        s = O.Scheme(dimension=2, subgrid=subgrid, boundary=bc, gamma=1.4, dx=1 / max(a, b))
+        grids.append(O.initial_average(_grid2d(a, b, 11 + k), s))
+    outs = []
+    for fuse in (False, True):
+        _say(f"[gpu] 2D sweep fused={fuse}: {subgrid}/{solver}/{precision}/{flattener}/{compat}, {shapes}")
+        plan = Plan(2, subgrid, solver, "ssprk(3,3)", probs, varmask=0x17, precision=precision,
+                    flattener=flattener, compat=compat, fuse2d=fuse)
+        plan.load(grids)
+        plan.reset()
+        st = plan.run(4)
+        outs.append((plan.state.clone(), [(s["steps"], s["failed"], s["t"]) for s in st]))
+        plan.close()
+    assert outs[0][1] == outs[1][1]
+    assert all(s[0] == 9 for s in outs[0][1])
+    assert torch.equal(outs[0][0], outs[1][0])
+
+
+@pytest.mark.parametrize("bc,parity", [("wrap", 0), ("edge", 1)])
+def test_fused_sweep_interface_fluxes(bc, parity):
+    """ah_evolve_space on a one-launch plan: the interface fluxes and eigmax
+    of both axes equal the two-launch plan's bit for bit."""
+    import torch
+    from astrea_amd.plan import Plan
+    n0, n1 = 45, 30
+    prob = dict(n0=n0, n1=n1, boundary=bc, dx=1 / n0, gamma=1.4, cfl=.4, t_end=1.)
+    s = O.Scheme(dimension=2, subgrid="ppm", boundary=bc, gamma=1.4, dx=1 / n0)
+    q0 = O.initial_average(_grid2d(n0, n1, 5), s)
+    res = []
+    for fuse in (False, True):
+        _say(f"[gpu] evolve_space fused={fuse} ppm/hllc {bc} {n0}x{n1} parity={parity}")
+        plan = Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x17, flattener="astrea", fuse2d=fuse)
+        plan.load([q0])
+        res.append(plan.evolve_space(parity))
+        plan.close()
+    for ax in (0, 1):
+        assert torch.equal(res[0][ax]["raw"], res[1][ax]["raw"])
+        assert res[0][ax]["eigmax"] == res[1][ax]["eigmax"]
+
+
+def test_fused_sweep_refused_where_not_built():
+    from astrea_amd import _lib as L
+    from astrea_amd.plan import Plan
+    prob = dict(n0=32, n1=32, boundary="wrap", dx=1 / 32, gamma=1.4, cfl=.4, t_end=1.)
+    for dims, sg, sol, vm in ((1, "ppm", "hllc", 0x13), (2, "weno5", "hllc", 0x17), (2, "ppm", "hlld", 0x77),
+                              (2, "pcm", "llf", 0x17)):
+        p1 = dict(prob, n1=1) if dims == 1 else prob
+        with pytest.raises(L.AstreaHipError, match="not implemented"):
+            Plan(dims, sg, sol, "ssprk(3,3)", [p1], varmask=vm, fuse2d=True)

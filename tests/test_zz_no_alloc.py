@@ -37,31 +37,36 @@ def _case(dims, n, seed):
     return w
 
 
-def _static_call(plan, parity):
-    """ah_evolve_space_static with the plan's scheme and problem on the
-    plan's state, into fresh output tensors."""
+def _static_call(plan, parity, fuse2d=False):
+    """ah_evolve_space_static with the plan's scheme (or its one-launch 2D
+    form) and problem on the plan's state, into fresh output tensors."""
     import torch
+    from astrea_amd import _lib as L
+    sch = L.AhScheme.from_buffer_copy(plan.scheme)
+    sch.fuse2d = 1 if fuse2d else 0
     p = plan.problems[0]
     f0 = torch.full((plan.nv, (p.n0 + 1) * p.n1), np.nan, dtype=torch.float64, device=plan.device)
     f1 = torch.full((plan.nv, (p.n1 + 1) * p.n0), np.nan, dtype=torch.float64, device=plan.device)
     eig = (ctypes.c_double * 2)()
-    rc = plan.lib.ah_evolve_space_static(ctypes.byref(plan.scheme), ctypes.byref(p), int(parity),
+    rc = plan.lib.ah_evolve_space_static(ctypes.byref(sch), ctypes.byref(p), int(parity),
                                          ctypes.c_void_p(plan.state.data_ptr()), ctypes.c_void_p(f0.data_ptr()),
                                          ctypes.c_void_p(f1.data_ptr()), eig, plan._stream())
     return rc, f0, f1, [eig[0], eig[1]]
 
 
-@pytest.mark.parametrize("dims,subgrid,solver,bc,sizes,flattener", [
-    (1, "ppm", "hllc", "edge", (4000, 700), "off"),
-    (2, "ppm", "hllc", "wrap", (160, 72), "astrea"),
-    (2, "plm", "llf", "edge", (97, 41), "off"),
+@pytest.mark.parametrize("dims,subgrid,solver,bc,sizes,flattener,fuse2d", [
+    (1, "ppm", "hllc", "edge", (4000, 700), "off", False),
+    (2, "ppm", "hllc", "wrap", (160, 72), "astrea", False),
+    (2, "plm", "llf", "edge", (97, 41), "off", False),
+    (2, "ppm", "hllc", "edge", (120, 53), "astrea", True),
 ])
-def test_static_evolve_space_equals_plan_entry(dims, subgrid, solver, bc, sizes, flattener):
+def test_static_evolve_space_equals_plan_entry(dims, subgrid, solver, bc, sizes, flattener, fuse2d):
     from astrea_amd import _lib as L
     from astrea_amd.plan import Plan
     for k, n in enumerate(sizes):
         for parity in ((0, 1) if dims == 2 else (0,)):
-            _say(f"[gpu] static evolve_space dims={dims} {subgrid}/{solver}/{bc} n={n} call={k} parity={parity}")
+            _say(f"[gpu] static evolve_space dims={dims} {subgrid}/{solver}/{bc} n={n} call={k} parity={parity}"
+                 f" fused={fuse2d}")
             s = O.Scheme(dimension=dims, subgrid=subgrid, boundary=bc, gamma=1.4, dx=1 / n)
             q0 = O.initial_average(_case(dims, n, 7 + k), s)
             prob = dict(n0=n, n1=n if dims == 2 else 1, boundary=bc, dx=1 / n, gamma=1.4, cfl=.4, t_end=1.)
@@ -69,7 +74,7 @@ def test_static_evolve_space_equals_plan_entry(dims, subgrid, solver, bc, sizes,
                         compat="physical", flattener=flattener)
             plan.load([q0])
             ref = plan.evolve_space(parity)
-            rc, f0, f1, eig = _static_call(plan, parity)
+            rc, f0, f1, eig = _static_call(plan, parity, fuse2d)
             assert rc == L.AH_OK, L.load().ah_status_str(rc)
             import torch
             assert torch.equal(f0, ref[0]["raw"])
