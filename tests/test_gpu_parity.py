@@ -816,6 +816,26 @@ def test_served_results_equal_alone():
         assert torch_equal(got, r.result), r.rid
 
 
+def test_served_results_equal_alone_at_stream_sizes():
+    """The same at the served stream's own sizes (128^2 and 256^2, PLM and
+    PPM): each request's state, step count and failure status equal its
+    solve alone."""
+    from astrea_amd import serve
+    reqs = serve.make_requests(6, 5, rate=40.0, sizes=(128, 256))
+    for r in reqs:
+        r.t_end = .01
+    _say(f"[gpu] serve equivalence at stream sizes: {[(r.n, r.subgrid) for r in reqs]}")
+    srv = serve.Server(hbm_budget=3 * 256 * 256 * 232, keep_results=True, chunk=4)
+    mix = srv.run(reqs, realtime=False)
+    res_mix = {r.rid: (r.result.clone(), dict(r.status)) for r in mix["done"]}
+    assert mix["rebuilds"] > 2 and len(res_mix) == len(reqs)
+    alone = serve.run_fifo(reqs, realtime=False, keep_results=True)
+    for r in alone["done"]:
+        got, st = res_mix[r.rid]
+        assert st["steps"] == r.status["steps"] and st["failed"] == r.status["failed"], (r.rid, st, r.status)
+        assert torch_equal(got, r.result), r.rid
+
+
 def torch_equal(a, b):
     import torch
     return bool(torch.equal(a, b))
