@@ -22,6 +22,7 @@ UNITS = {
     "ah_fp64_1d.hip": ["-ffp-contract=off"],
     "ah_fp64_2d.hip": ["-ffp-contract=off"],
     "ah_fp64_2d_fused.hip": ["-ffp-contract=off"],
+    "ah_fp64_2d_author.hip": ["-ffp-contract=off"],
     "ah_fp32_1d.hip": [],
     "ah_fp32_2d.hip": [],
     "ah_fp32_2d_fused.hip": [],

@@ -56,9 +56,6 @@ std::vector<Stage> rk_program(int rk) {
     return {};
 }
 
-// PPM authors c/ph (ppm.py:43-62): 1D hydrodynamics (whose zero components
-// make the extrapolant limiter's grid-wide test always true), FP64, without
-// the flattener and artificial viscosity the reference ties to author 'mc'
 // the one-launch 2D sweep: hydrodynamic PLM/PPM with LLF or HLLC
 bool fused_supported(const ah_scheme* sch) {
     return sch->dims == 2 && sch->varmask == AH_VARS_H2 && !sch->magnetic_2d && sch->tile_s == 0 &&
@@ -66,10 +63,15 @@ bool fused_supported(const ah_scheme* sch) {
            (sch->solver == AH_LLF || sch->solver == AH_HLLC);
 }
 
+// PPM authors c/ph (ppm.py:43-62): 1D and 2D hydrodynamics (whose zero
+// components make the extrapolant limiter's grid-wide test always true), FP64,
+// without the flattener and artificial viscosity the reference ties to author
+// 'mc', on the default tiles
 bool author_supported(const ah_scheme* sch) {
-    return (sch->ppm_author == AH_PPM_C || sch->ppm_author == AH_PPM_PH) && sch->dims == 1 &&
-           sch->recon == AH_PPM && sch->varmask == AH_VARS_H1 && sch->precision == AH_FP64 &&
-           sch->flattener == AH_FLAT_OFF && !sch->av && !sch->magnetic_2d;
+    return (sch->ppm_author == AH_PPM_C || sch->ppm_author == AH_PPM_PH) &&
+           ((sch->dims == 1 && sch->varmask == AH_VARS_H1) || (sch->dims == 2 && sch->varmask == AH_VARS_H2)) &&
+           sch->recon == AH_PPM && sch->precision == AH_FP64 && sch->flattener == AH_FLAT_OFF && !sch->av &&
+           !sch->magnetic_2d && !sch->fuse2d && sch->tile_s == 0;
 }
 
 int popcount8(int m) {
@@ -238,7 +240,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     // and its 2D form cannot run (ppm.py:149-154 adds arrays of different shapes)
     if (sch->av && (sch->dims != 1 || sch->recon != AH_PPM)) return AH_ERR_UNSUPPORTED;
     if (sch->magnetic_2d && (sch->dims != 2 || !(sch->varmask & 0x60) || sch->rk == AH_SSPRK104 ||
-                             sch->flattener != AH_FLAT_OFF || sch->precision != AH_FP64))
+                             sch->flattener != AH_FLAT_OFF))
         return AH_ERR_UNSUPPORTED;
     ah_plan* p = new (std::nothrow) ah_plan();
     if (!p) return AH_ERR_NOMEM;
@@ -262,7 +264,8 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
                                         : kernels_1d_small_fp32(sch->varmask, sch->recon, sch->solver, &p->ks)) &&
              p->ks.ts1 == sch->tile_s;
     else if (sch->ppm_author != AH_PPM_MC)
-        ok = author_supported(sch) && kernels_1d_author_fp64(sch->ppm_author, sch->solver, &p->ks);
+        ok = author_supported(sch) && (sch->dims == 1 ? kernels_1d_author_fp64(sch->ppm_author, sch->solver, &p->ks)
+                                                       : kernels_2d_author_fp64(sch->ppm_author, sch->solver, &p->ks));
     else if (sch->precision == AH_FP64)
         ok = sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks)
                             : kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &p->ks);
@@ -551,6 +554,12 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
         a.tiles = p->tl[1 + ax].d;
         a.ntiles_dev = p->tl[1 + ax].d_count;
         a.wF = p->sch.magnetic_2d ? p->wF[ax] : nullptr;
+        if (p->ks.recon[ax] && p->sch.ppm_author != AH_PPM_MC) {
+            // the interface-limiter tests of this sweep-stage over every cell first
+            SweepArgs f = a;
+            f.flags_only = 1;
+            p->ks.recon[ax](f, blocks_for(p, p->tl[1 + ax]), s);
+        }
         if (p->ks.recon[ax]) {
             ProfScope ps_(p, ax == 0 ? AH_K_RECON0 : AH_K_RECON1, s);
             p->ks.recon[ax](a, blocks_for(p, p->tl[1 + ax]), s);
@@ -1213,7 +1222,8 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
                                                : kernels_1d_small_fp32(sch->varmask, sch->recon, sch->solver, &ks)) &&
                         ks.ts1 == sch->tile_s
                   : sch->ppm_author != AH_PPM_MC
-                  ? kernels_1d_author_fp64(sch->ppm_author, sch->solver, &ks)
+                  ? (sch->dims == 1 ? kernels_1d_author_fp64(sch->ppm_author, sch->solver, &ks)
+                                    : kernels_2d_author_fp64(sch->ppm_author, sch->solver, &ks))
                   : sch->precision == AH_FP64
                   ? (sch->dims == 1 ? kernels_1d_fp64(sch->varmask, sch->recon, sch->solver, &ks)
                                     : kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &ks))
@@ -1334,6 +1344,11 @@ int ah_evolve_space_static(const ah_scheme* sch, const ah_problem* q, int parity
         for (int ax = 0; ax < 2; ++ax) {
             a.tiles = spec[ax].dst;
             a.ntiles_dev = spec[ax].count;
+            if (ks.recon[ax] && sch->ppm_author != AH_PPM_MC) {
+                SweepArgs f = a;
+                f.flags_only = 1;
+                ks.recon[ax](f, blocks(ax), s);
+            }
             if (ks.recon[ax]) ks.recon[ax](a, blocks(ax), s);
             a.tiles = spec[2 + ax].dst;
             a.ntiles_dev = spec[2 + ax].count;

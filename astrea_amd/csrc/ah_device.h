@@ -621,6 +621,9 @@ __device__ inline V8<double> flux_hllc_split(const Iface<double>& s, int sax, do
 // HLLD (solvers.py:142-232), evaluated on all 8 components exactly as the
 // reference writes it (dead components enter as zeros).  SAX = solver axis;
 // Bn = normal field of the right cell's centred state (solvers.py:168,183).
+// FP64 on the split path too: its region selection is discontinuous (both
+// Alfven speeds s_M - B_n / sqrt(rho*), the default F(w-)), and FP32 fast
+// speeds flipped a region of the Brio-Wu problem within six steps.
 template <class R, int MASK, int SAX>
 __device__ inline V8<R> flux_hlld(const Iface<R>& s, R Bn, R gamma) {
     constexpr int A = SAX % 3, O = (SAX + 1) % 3, C = (SAX + 2) % 3;

@@ -53,7 +53,9 @@ inline void set2d(KernelSet* ks) {
     ks->fts[1] = FTS1;
     ks->ftt[1] = FTT1;
     ks->real_bytes = sizeof(R);
-    if constexpr (VS<MASK>::MHD && !SPLIT) {
+    // constrained transport: the same FP64 kernels on either path (the split
+    // path's FP32 part is in the sweeps' wave speeds)
+    if constexpr (VS<MASK>::MHD) {
         constexpr int TTS = 64, TTT = 8;
         ks->tts = TTS;
         ks->ttt = TTT;
@@ -140,6 +142,20 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
         }
     }
     return false;
+}
+
+// 2D hydrodynamic PPM with author c or ph (FP64): set2d's kernels and tiles
+// (32 x 8 and 64 x 4 reconstruction tiles for four FP64 components), the
+// reconstruction launches running the author's limiters
+template <int AUTH, int SOLVER>
+inline void set2d_author(KernelSet* ks) {
+    set2d<double, 0x17, PPM, SOLVER, false>(ks);
+    ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
+        launch_sweep(recon2d_kernel<double, 0x17, PPM, 0, 32, 8, false, AUTH>, a, nb, s);
+    };
+    ks->recon[1] = [](const SweepArgs& a, int nb, hipStream_t s) {
+        launch_sweep(recon2d_kernel<double, 0x17, PPM, 1, 64, 4, false, AUTH>, a, nb, s);
+    };
 }
 
 // 2D hydrodynamics, one launch per sweep: the flux kernel reconstructs its

@@ -83,7 +83,7 @@ struct SweepArgs {
     int compat;           // 0: reference solver axis (enumeration index), 1: physical axis
     int av;               // 1D PPM: add the artificial-viscosity flux alpha nu (q_k - q_{k-1})
     int stage_idx;        // stage of the step (PPM authors c/ph read that stage's limiter tests)
-    int flags_only;       // 1D PPM authors c/ph: only record this stage's interface-limiter tests
+    int flags_only;       // PPM authors c/ph: only record this stage's interface-limiter tests
     // 2D second sweep of the middle SSPRK(3,3) stage: write K2 = (3U + K1 + dt L) / 4
     // (evolvers.py:172) in place of L, in the combine kernel's operation order
     double* fuse_k2;      // null: write L
@@ -837,7 +837,11 @@ struct RowAcc {
 // instantiations spill 80 bytes per lane to do so and still run 10 % faster
 // than at three waves on the 1024^2 shear layer).  MHD windows leave LDS for
 // three, so their registers stay uncapped.
-template <class R, int MASK, int RECON, int AX, int TS, int TT, bool SPLIT = false>
+// AUTH (PPM, 2D hydrodynamics, FP64): author 'c' or 'ph' (ppm.py:43-71).  A
+// first launch with flags_only records the stage's grid-wide interface-limiter
+// tests of this sweep (limiters.py:58) in the problem's aflags, bits
+// 2 (2 stage + AX) and 2 (2 stage + AX) + 1; the second reconstructs with them.
+template <class R, int MASK, int RECON, int AX, int TS, int TT, bool SPLIT = false, int AUTH = AUTH_MC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VS<MASK>::NV * (int)sizeof(R) > 32 ? 1 : 4)))
 void recon2d_kernel(SweepArgs a) {
     using V = VS<MASK>;
@@ -854,6 +858,8 @@ void recon2d_kernel(SweepArgs a) {
     // PPM flattener: chi~ of each own row at positions [s0-2, s0+TS]
     constexpr int NCHI = TS + 3;
     __shared__ R chi_s[RECON == PPM ? TT : 1][RECON == PPM ? NCHI : 1];
+    constexpr bool AU = RECON == PPM && AUTH != AUTH_MC;
+    __shared__ unsigned abits;  // authors c/ph: the tile's interface-limiter tests
 
     const int ntiles_ = wave_uniform(a.ntiles_dev ? *a.ntiles_dev : a.ntiles);
     for (int ti_ = blockIdx.x; ti_ < ntiles_; ti_ += gridDim.x) {
@@ -861,6 +867,7 @@ void recon2d_kernel(SweepArgs a) {
     const ProbDesc pd = ld_const(a.pd + tile.prob);
     ProbState* ps = a.ps + tile.prob;
     if (!ld_const(&ps->active)) continue;
+    if (AU && threadIdx.x == 0) abits = 0u;  // read after the barriers of stages A and B
     const int ns = AX == 0 ? pd.n0 : pd.n1, nt = AX == 0 ? pd.n1 : pd.n0;
     const int bc = pd.bc, s0 = tile.s0, t0 = tile.t0;
     const double gamma = pd.gamma;
@@ -936,6 +943,39 @@ void recon2d_kernel(SweepArgs a) {
     const bool clampS = bc == EDGE && !(s0 >= HS && s0 + TS + HS <= ns);
     auto cells = [&](auto clamp_tag) {
     constexpr bool CL = decltype(clamp_tag)::value;
+    if constexpr (AU) {
+        const int sh = 2 * (2 * a.stage_idx + AX);
+        if (a.flags_only) {
+            unsigned bits = 0;
+            for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
+                const int it = i / TS, s = s0 + i % TS, t = t0 + it;
+                if (s >= ns || t >= nt) continue;
+                const RowAcc<R, MASK, TT, NWS, CL> W{ws, it, bws, ns};
+                bits |= author_face_tests<R, MASK, AUTH>(W, s);
+            }
+            if (bits) atomicOr(&abits, bits);
+            __syncthreads();
+            if (threadIdx.x == 0 && abits) atomicOr(&ps->aflags, (unsigned long long)abits << sh);
+            return;
+        }
+        const unsigned long long af = ld_const(&ps->aflags) >> sh;
+        const bool anyR = (af & 1ull) != 0, anyL = (af & 2ull) != 0;
+        for (int i = threadIdx.x; i < TT * TS; i += blockDim.x) {
+            const int it = i / TS, s = s0 + i % TS, t = t0 + it;
+            if (s >= ns || t >= nt) continue;
+            const RowAcc<R, MASK, TT, NWS, CL> W{ws, it, bws, ns};
+            V8<R> L, Rr;
+            author_cell<R, MASK, AUTH>(W, s, anyR, anyL, L, Rr);
+            const long long o = fo + (long long)t * ns + s;
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                if (V::live(c)) {
+                    wLo[V::slot(c) * P + o] = L.c[c];
+                    wRo[V::slot(c) * P + o] = Rr.c[c];
+                }
+        }
+        return;
+    }
     if constexpr (RECON == PPM) {
         // each cell's face etas read chi~ at me-1, me, me+1 (me = e(ce-1),
         // e(ce)): one evaluation per position instead of up to four per cell

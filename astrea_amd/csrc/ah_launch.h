@@ -36,8 +36,9 @@ bool kernels_1d_fp32(int mask, int recon, int solver, KernelSet* ks);
 // 1D hydrodynamic PLM/PPM with LLF or HLLC on the 115-cell tiles
 bool kernels_1d_small_fp64(int mask, int recon, int solver, KernelSet* ks);
 bool kernels_1d_small_fp32(int mask, int recon, int solver, KernelSet* ks);
-// 1D hydrodynamic PPM with author c (1) or ph (2), FP64, LLF or HLLC
+// 1D / 2D hydrodynamic PPM with author c (1) or ph (2), FP64, LLF or HLLC
 bool kernels_1d_author_fp64(int author, int solver, KernelSet* ks);
+bool kernels_2d_author_fp64(int author, int solver, KernelSet* ks);
 bool kernels_2d_fp32(int mask, int recon, int solver, KernelSet* ks);
 // 2D hydrodynamic PLM/PPM with LLF or HLLC, one launch per sweep (recon[] null)
 bool kernels_2d_fused_fp64(int recon, int solver, KernelSet* ks);

@@ -69,10 +69,11 @@ enum ah_rk {
     AH_SSPRK53 = 5, AH_SSPRK54 = 6, AH_SSPRK104 = 7
 };
 /* AH_FP64: every operation in FP64, in the reference's order (bitwise the numpy
- * reference).  AH_SPLIT: the Riemann solvers' coefficient algebra (wave-speed
- * estimates, star-region factors), the spectral radius and the reciprocal of the
- * primitive recovery in FP32; states, interface states, reconstruction, fluxes,
- * stage combinations and the dt reduction in FP64.  Not bitwise. */
+ * reference).  AH_SPLIT: the HLLC solver's coefficient algebra (wave-speed estimates,
+ * star-region factors), the spectral radius and the reciprocal of the primitive
+ * recovery in FP32; states, interface states, reconstruction, fluxes, HLLD,
+ * constrained transport, stage combinations and the dt reduction in FP64.  Not
+ * bitwise. */
 enum ah_precision { AH_FP64 = 0, AH_SPLIT = 1 };
 /* AH_COMPAT_ASTREA reproduces the reference's Riemann-solver axis argument:
  * the enumeration index of the sweep, which swaps the normal velocity of
@@ -108,8 +109,8 @@ typedef struct ah_scheme {
                         whose output the reference computes but never applies: 0 is the
                         reference-equivalent setting) */
     int ppm_author;  /* ah_ppm_author (ppm.py:13; the reference's evolve_space always passes 'mc',
-                        evolvers.py:17); c / ph: 1D hydrodynamics in FP64 without flattener or
-                        artificial viscosity */
+                        evolvers.py:17); c / ph: 1D and 2D hydrodynamics in FP64 without
+                        flattener or artificial viscosity, on the default tiles */
     int fuse2d;      /* 2D hydrodynamics (PLM/PPM with LLF/HLLC): 1 = one launch per sweep, the
                         interface states reconstructed inside each flux tile instead of written to
                         and read back from HBM by a reconstruction launch; the same results bit for

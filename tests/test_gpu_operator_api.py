@@ -426,14 +426,53 @@ def test_ppm_author_trajectory(author, bc, solver):
     assert not np.array_equal(ref.grid, O.run(q0, replace(s, ppm_author="mc"), 1e9, max_steps=20).grid)
 
 
-def test_ppm_author_refused_outside_1d_hydro():
+@pytest.mark.parametrize("author", ["c", "ph"])
+@pytest.mark.parametrize("bc,solver", [("edge", "hllc"), ("wrap", "llf")])
+def test_ppm_author_trajectory_2d(author, bc, solver):
+    """2D PPM with author 'c' or 'ph' (the grid-wide tests recorded per sweep
+    and stage): evolve_space fluxes of both axes and 10 steps of a sheared
+    two-state problem equal the oracle."""
+    from astrea_amd.plan import Plan
+    n = 40
+    X, Y = np.meshgrid((np.arange(n) + .5) / n, (np.arange(n) + .5) / n, indexing="ij")
+    w = np.zeros((n, n, 8))
+    w[..., 0] = np.where((X > .5) ^ (Y > .4), 1.5, .4) * (1 + .1 * np.sin(6 * np.pi * Y))
+    w[..., 1] = np.where(Y > .4, .3, -.2)
+    w[..., 2] = .1 * np.sin(2 * np.pi * X)
+    w[..., 4] = np.where(X > .5, 1., .15)
+    s = O.Scheme(dimension=2, subgrid="ppm", solver=solver, boundary=bc, gamma=1.4, cfl=.4, dx=1 / n,
+                 ppm_author=author, compat="physical")
+    q0 = O.initial_average(w, s)
+    _say(f"[gpu] 2D ppm author {author} {bc} {solver}")
+    prob = dict(n0=n, n1=n, boundary=bc, dx=1 / n, gamma=1.4, cfl=.4, t_end=1e9, max_steps=10)
+    plan = Plan(2, "ppm", solver, "ssprk(3,3)", [prob], varmask=0x17, ppm_author=author, compat="physical")
+    plan.load([q0])
+    fl = plan.evolve_space(0)
+    ref = O.evolve_space(q0, s)
+    for ax in (0, 1):
+        assert _scaled_err(fl[ax]["flux"], ref[ax]["flux"]) <= 1e-14
+    plan.reset()
+    st = plan.run(32)[0]
+    out = plan.unpack(0)
+    plan.close()
+    r = O.run(q0, s, 1e9, max_steps=10)
+    assert st["steps"] == r.steps == 10 and not st["failed"]
+    e = float(np.max(rel_l1(out, r.grid)))
+    _say(f"[gpu]   rel L1 vs oracle {e:.1e}")
+    assert e <= 1e-12
+    assert not np.array_equal(r.grid, O.run(q0, replace(s, ppm_author="mc"), 1e9, max_steps=10).grid)
+
+
+def test_ppm_author_refused_outside_hydro_fp64():
     from astrea_amd import _lib as L
     from astrea_amd.plan import Plan
     prob = dict(n0=16, n1=16, boundary="wrap", dx=1 / 16, gamma=1.4, cfl=.4, t_end=1.)
-    with pytest.raises(L.AstreaHipError, match="not implemented"):
-        Plan(2, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x17, ppm_author="c")
-    with pytest.raises(L.AstreaHipError, match="not implemented"):
-        Plan(1, "ppm", "hllc", "ssprk(3,3)", [prob], varmask=0x13, ppm_author="ph", flattener="astrea")
+    for dims, sg, kw in ((2, "ppm", dict(varmask=0x17, precision="split")), (2, "weno5", dict(varmask=0x17)),
+                         (2, "ppm", dict(varmask=0x77, magnetic_2d=True)), (2, "ppm", dict(varmask=0x17, fuse2d=True)),
+                         (1, "ppm", dict(varmask=0x13, flattener="astrea"))):
+        with pytest.raises(L.AstreaHipError, match="not implemented"):
+            Plan(dims, sg, "hllc" if not kw.get("magnetic_2d") else "hlld", "ssprk(3,3)", [prob], ppm_author="ph",
+                 **kw)
 
 
 # ---- tile shapes of the 1D sweep (the autotuner's choices) -----------------------

@@ -526,15 +526,16 @@ def test_shear_layer_physical_compat(subgrid, flattener):
     assert e <= 1e-12
 
 
-def _split_vs_fp64(dims, recon, prob, q0, steps=None, flattener="off", varmask=0x17):
+def _split_vs_fp64(dims, recon, prob, q0, steps=None, flattener="off", varmask=0x17, solver="hllc",
+                   magnetic_2d=False):
     """Run the FP64 and split paths from one initial state; rel L1 of split
     against FP64 (max over variables) and both statuses."""
     from astrea_amd.plan import Plan
     outs, sts = {}, {}
     pr = dict(prob, max_steps=steps) if steps else prob
     for prec in ("fp64", "split"):
-        plan = Plan(dims, recon, "hllc", "ssprk(3,3)", [pr], flattener=flattener, varmask=varmask,
-                    compat="physical", precision=prec)
+        plan = Plan(dims, recon, solver, "ssprk(3,3)", [pr], flattener=flattener, varmask=varmask,
+                    compat="physical", precision=prec, magnetic_2d=magnetic_2d)
         plan.load([q0])
         plan.reset()
         sts[prec] = plan.run(chunk=64)[0]
@@ -555,6 +556,41 @@ def test_split_precision_shear_layer():
                            _shear(n), steps=10, flattener="astrea")
     _say(f"[gpu]   rel L1 {e:.2e}")
     assert st["split"]["steps"] == 10 and not st["split"]["failed"]
+    assert e <= 1e-5
+
+
+def test_split_precision_orszag_tang():
+    """Split path of 2D MHD (FP32 spectral radius and primitive-recovery
+    reciprocal; HLLD, constrained transport and values FP64) on the
+    Orszag-Tang vortex, 128^2, 12 steps: rel L1 against FP64 <= 1e-5."""
+    from astrea_amd import ic
+    n = 128
+    _say(f"[gpu] split vs fp64: Orszag-Tang {n}^2, WENO5/HLLD/CT, 12 steps")
+    import torch
+    q0 = ic.initial_average(torch.from_numpy(ic.orszag_tang(n)), 5 / 3, "wrap", True).numpy()
+    e, st = _split_vs_fp64(2, "weno5", dict(n0=n, n1=n, boundary="wrap", dx=1 / n, gamma=5 / 3, cfl=.4,
+                                            t_end=1e9), q0, steps=12, varmask=0x77, solver="hlld",
+                           magnetic_2d=True)
+    _say(f"[gpu]   rel L1 {e:.2e}")
+    assert st["split"]["steps"] == 12 and not st["split"]["failed"]
+    assert e <= 1e-5
+
+
+def test_split_precision_brio_wu():
+    """Split path of 1D MHD with HLLD on Brio-Wu, 400 cells, 40 steps: rel
+    L1 against FP64 <= 1e-5 (with FP32 fast speeds inside HLLD this case
+    left FP64 by 0.1 at the discontinuity after six steps: a region switch
+    of the reference's HLLD flipped, so HLLD stays FP64 on the split path)."""
+    from astrea_amd import ic
+    from astrea_amd.options import handle_variables
+    sv = handle_variables({"config": "brio-wu", "cells": 400, "solver": "hlld", "subgrid": "plm",
+                           "timestep": "ssprk(3,3)"}, t_end=.1)
+    g = np.asarray(ic.initialise(sv))
+    _say("[gpu] split vs fp64: Brio-Wu 400 cells, PLM/HLLD, 40 steps")
+    e, st = _split_vs_fp64(1, "plm", dict(n0=400, boundary=sv.boundary, dx=sv.dx, gamma=sv.gamma, cfl=.5,
+                                          t_end=1.), g, steps=40, varmask=0xFF, solver="hlld")
+    _say(f"[gpu]   steps {st['split']['steps']} failed {st['fp64']['failed']}/{st['split']['failed']} rel L1 {e:.2e}")
+    assert st["fp64"]["steps"] == st["split"]["steps"] == 40 and not st["split"]["failed"]
     assert e <= 1e-5
 
 
