@@ -572,7 +572,8 @@ def bench_mhd(args):
             if p != "fp64":
                 num = (r["state"] - b).abs().sum(dim=1)
                 l1[p] = float(torch.max(num / b.abs().sum(dim=1)).item())
-    ok = [p for p in res if (l1[p] <= 1e-5 if p in l1 else p == "fp64")]
+    # without an FP64 run (a profiling pass of one path) the fastest path run reports
+    ok = [p for p in res if (l1[p] <= 1e-5 if p in l1 else p == "fp64")] or list(res)
     best = min(ok, key=lambda p: res[p]["ms"])
     r = res[best]
     ms, prof, div0, div1 = r["ms"], r["prof"], r["div0"], r["div1"]
@@ -584,16 +585,19 @@ def bench_mhd(args):
     per_launch = b_alg_zu * n * n / 6
     achieved = per_launch / (dom_ms / 1e3) / 1e9
     # HBM bytes and VALU instructions of that launch from the counter runs
-    # summarised in profiles/ (2048^2, FP64 only)
+    # summarised in profiles/ (2048^2, per path)
     traffic, valu, tsrc = None, None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_ot2048.json")
-    if n == 2048 and best == "fp64" and os.path.exists(tpath):
+    key = "" if best == "fp64" else "_" + best
+    if n == 2048 and os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        traffic, tsrc = tj.get("bytes_per_launch"), tj.get("source")
-        issue_s = tj["valu_wave_insts_per_launch"] * 4 / (1024 * 2.4e9)
-        valu = {"wave_insts_per_launch": tj["valu_wave_insts_per_launch"], "issue_ms": issue_s * 1e3,
-                "frac": issue_s / (dom_ms / 1e3), "source": tsrc}
+        traffic, tsrc = tj.get("bytes_per_launch" + key), tj.get("source")
+        insts = tj.get("valu_wave_insts_per_launch" + key)
+        if insts:
+            issue_s = insts * 4 / (1024 * 2.4e9)
+            valu = {"wave_insts_per_launch": insts, "issue_ms": issue_s * 1e3, "frac": issue_s / (dom_ms / 1e3),
+                    "source": tsrc}
     cpu = None if args.no_cpu else cpu_baseline(128, 8, "mhd")
     return {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -737,7 +741,7 @@ def main():
             den = b.abs().sum(dim=1)
             l1[path] = float(torch.max(num / den).item())
     # a path qualifies by its L1 against fp64, or (run without fp64) by being an FP64 path
-    ok_paths = [p for p in results if (l1[p] <= 1e-5 if p in l1 else p.startswith("fp64"))]
+    ok_paths = [p for p in results if (l1[p] <= 1e-5 if p in l1 else p.startswith("fp64"))] or list(results)
     best = min(ok_paths, key=lambda p: results[p]["ms"])
     r = results[best]
     zu = n * n * args.steps
