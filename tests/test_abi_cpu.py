@@ -33,6 +33,30 @@ def test_library_loads_and_exports_every_symbol():
     assert L.load().ah_status_str(-2).decode().startswith("option combination")
 
 
+def header_struct_fields(name):
+    """Field names of ``typedef struct name {...}`` in the header, in order
+    (arrays named without their extent)."""
+    txt = open(os.path.join(ROOT, "include", "astrea_hip.h")).read()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), txt, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        # "long long max_steps", "int n0, n1", "int reserved[1]": the type, then declarators
+        first = re.match(r"((?:const\s+)?(?:unsigned\s+)?(?:long\s+long|int|double|float|char)\s*\**)\s*(.*)", decl, re.S)
+        names += [re.sub(r"\[\d+\]", "", d).strip(" *") for d in first.group(2).split(",")]
+    return names
+
+
+@pytest.mark.parametrize("cname,py", [("ah_scheme", "AhScheme"), ("ah_problem", "AhProblem"),
+                                      ("ah_problem_status", "AhProblemStatus")])
+def test_struct_fields_follow_the_header(cname, py):
+    """The ctypes mirrors name the header's fields in the header's order."""
+    assert [f[0] for f in getattr(L, py)._fields_] == header_struct_fields(cname)
+
+
 def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(L.AhScheme) == 16 * 4
     assert ctypes.sizeof(L.AhProblem) == 4 * 4 + 4 * 8 + 8
