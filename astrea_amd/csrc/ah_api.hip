@@ -272,6 +272,19 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     else
         ok = sch->dims == 1 ? kernels_1d_fp32(sch->varmask, sch->recon, sch->solver, &p->ks)
                             : kernels_2d_fp32(sch->varmask, sch->recon, sch->solver, &p->ks);
+    if (ok && sch->magnetic_2d && sch->precision != AH_FP64) {
+        // the split path's constrained transport runs the FP64 kernels
+        KernelSet ct;
+        ok = kernels_2d_fp64(sch->varmask, sch->recon, sch->solver, &ct);
+        for (int ax = 0; ax < 2; ++ax) p->ks.trecon[ax] = ct.trecon[ax];
+        p->ks.corner = ct.corner;
+        p->ks.magL = ct.magL;
+        p->ks.bface = ct.bface;
+        p->ks.refine = ct.refine;
+        p->ks.copyB = ct.copyB;
+        p->ks.tts = ct.tts;
+        p->ks.ttt = ct.ttt;
+    }
     p->prog = rk_program(sch->rk);
     if (sch->magnetic_2d && !p->ks.corner) ok = false;
     if (!ok || p->prog.empty()) {

@@ -3,10 +3,9 @@
 
 namespace ah {
 bool kernels_2d_author_fp64(int author, int solver, KernelSet* ks) {
-    if (author == AUTH_C && solver == LLF) set2d_author<AUTH_C, LLF>(ks);
-    else if (author == AUTH_C && solver == HLLC) set2d_author<AUTH_C, HLLC>(ks);
-    else if (author == AUTH_PH && solver == LLF) set2d_author<AUTH_PH, LLF>(ks);
-    else if (author == AUTH_PH && solver == HLLC) set2d_author<AUTH_PH, HLLC>(ks);
+    if ((solver != LLF && solver != HLLC) || !kernels_2d_fp64(0x17, PPM, solver, ks)) return false;
+    if (author == AUTH_C) set2d_author<AUTH_C>(ks);
+    else if (author == AUTH_PH) set2d_author<AUTH_PH>(ks);
     else return false;
     return true;
 }

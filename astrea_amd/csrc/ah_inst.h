@@ -53,9 +53,10 @@ inline void set2d(KernelSet* ks) {
     ks->fts[1] = FTS1;
     ks->ftt[1] = FTT1;
     ks->real_bytes = sizeof(R);
-    // constrained transport: the same FP64 kernels on either path (the split
-    // path's FP32 part is in the sweeps' wave speeds)
-    if constexpr (VS<MASK>::MHD) {
+    // constrained transport: FP64 translation units only (a split plan takes
+    // these launchers from the FP64 kernel set; an instantiation here would
+    // give the split unit's copy the same symbol, compiled with contraction)
+    if constexpr (VS<MASK>::MHD && !SPLIT) {
         constexpr int TTS = 64, TTT = 8;
         ks->tts = TTS;
         ks->ttt = TTT;
@@ -144,12 +145,11 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
     return false;
 }
 
-// 2D hydrodynamic PPM with author c or ph (FP64): set2d's kernels and tiles
-// (32 x 8 and 64 x 4 reconstruction tiles for four FP64 components), the
-// reconstruction launches running the author's limiters
-template <int AUTH, int SOLVER>
+// 2D hydrodynamic PPM with author c or ph (FP64): replaces the reconstruction
+// launchers of the FP64 kernel set (set2d's 32 x 8 and 64 x 4 tiles for four
+// FP64 components) with the author's; the flux launchers stay
+template <int AUTH>
 inline void set2d_author(KernelSet* ks) {
-    set2d<double, 0x17, PPM, SOLVER, false>(ks);
     ks->recon[0] = [](const SweepArgs& a, int nb, hipStream_t s) {
         launch_sweep(recon2d_kernel<double, 0x17, PPM, 0, 32, 8, false, AUTH>, a, nb, s);
     };

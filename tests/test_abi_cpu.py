@@ -114,3 +114,29 @@ def test_gamma_one_takes_the_epsilon_of_the_stated_precision():
     assert O.handle_variables({"gamma": 1.0}).gamma == 1 + float(np.finfo("float64").eps)
     assert O.handle_variables({"gamma": 1.0, "precision": "float16"}).gamma == 1 + float(np.finfo("float16").eps)
     assert O.handle_variables({"gamma": 1}).gamma == 1.4      # an int is not a valid gamma (generic.py:188-192)
+
+
+def test_no_kernel_is_instantiated_under_two_fp_modes():
+    """A kernel template instantiated in two translation units keeps one
+    symbol in the library, and the launch runs whichever copy the loader
+    kept.  Units built with different floating-point flags (FP64 units with
+    -ffp-contract=off, split units with contraction) must therefore not
+    instantiate the same kernel, or an FP64 launch could run a contracted
+    copy and lose its bitwise parity."""
+    import shutil
+    import subprocess
+    from astrea_amd import _build
+    if not os.path.isdir(_build.OBJ) or shutil.which("nm") is None:
+        pytest.skip("objects not built (run python -m astrea_amd._build)")
+    where = {}
+    for unit, flags in _build.UNITS.items():
+        obj = os.path.join(_build.OBJ, unit.replace(".hip", ".o"))
+        if not os.path.exists(obj):
+            pytest.skip(f"{obj} missing")
+        out = subprocess.run(["nm", "--defined-only", obj], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            sym = line.split()[-1]
+            if sym.startswith("_ZN2ah") and "kernel" in sym:
+                where.setdefault(sym, set()).add(tuple(flags))
+    mixed = sorted(s for s, f in where.items() if len(f) > 1)
+    assert not mixed, mixed[:5]
