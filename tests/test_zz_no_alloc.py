@@ -54,24 +54,25 @@ def _static_call(plan, parity, fuse2d=False):
     return rc, f0, f1, [eig[0], eig[1]]
 
 
-@pytest.mark.parametrize("dims,subgrid,solver,bc,sizes,flattener,fuse2d", [
-    (1, "ppm", "hllc", "edge", (4000, 700), "off", False),
-    (2, "ppm", "hllc", "wrap", (160, 72), "astrea", False),
-    (2, "plm", "llf", "edge", (97, 41), "off", False),
-    (2, "ppm", "hllc", "edge", (120, 53), "astrea", True),
+@pytest.mark.parametrize("dims,subgrid,solver,bc,sizes,flattener,fuse2d,author", [
+    (1, "ppm", "hllc", "edge", (4000, 700), "off", False, "mc"),
+    (2, "ppm", "hllc", "wrap", (160, 72), "astrea", False, "mc"),
+    (2, "plm", "llf", "edge", (97, 41), "off", False, "mc"),
+    (2, "ppm", "hllc", "edge", (120, 53), "astrea", True, "mc"),
+    (2, "ppm", "llf", "wrap", (64, 40), "off", False, "ph"),
 ])
-def test_static_evolve_space_equals_plan_entry(dims, subgrid, solver, bc, sizes, flattener, fuse2d):
+def test_static_evolve_space_equals_plan_entry(dims, subgrid, solver, bc, sizes, flattener, fuse2d, author):
     from astrea_amd import _lib as L
     from astrea_amd.plan import Plan
     for k, n in enumerate(sizes):
         for parity in ((0, 1) if dims == 2 else (0,)):
             _say(f"[gpu] static evolve_space dims={dims} {subgrid}/{solver}/{bc} n={n} call={k} parity={parity}"
-                 f" fused={fuse2d}")
+                 f" fused={fuse2d} author={author}")
             s = O.Scheme(dimension=dims, subgrid=subgrid, boundary=bc, gamma=1.4, dx=1 / n)
             q0 = O.initial_average(_case(dims, n, 7 + k), s)
             prob = dict(n0=n, n1=n if dims == 2 else 1, boundary=bc, dx=1 / n, gamma=1.4, cfl=.4, t_end=1.)
             plan = Plan(dims, subgrid, solver, "ssprk(3,3)", [prob], varmask=0x13 if dims == 1 else 0x17,
-                        compat="physical", flattener=flattener)
+                        compat="physical", flattener=flattener, ppm_author=author)
             plan.load([q0])
             ref = plan.evolve_space(parity)
             rc, f0, f1, eig = _static_call(plan, parity, fuse2d)
