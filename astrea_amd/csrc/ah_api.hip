@@ -643,7 +643,10 @@ static MhdArgs mhd_args(ah_plan* p) {
     m.ps = p->d_ps;
     m.tiles = p->tl[5].d;
     m.ntiles_dev = p->tl[5].d_count;
-    m.solver_hll = p->sch.solver != AH_LLF;
+    // corner signal speeds (mag_field.py:135-159): the HLL estimate for the
+    // HLL family (solver_category 'hll'), the Jacobian's extreme eigenvalues
+    // for every other solver
+    m.solver_hll = p->sch.solver == AH_HLLC || p->sch.solver == AH_HLLD;
     return m;
 }
 

@@ -898,6 +898,7 @@ __device__ inline void jacobi_svd8(double A[8][8], double V[8][8]) {
 template <int MASK>
 __device__ inline V8<double> flux_dots(const Iface<double>& s, int sax, double gamma) {
     double xp[8], xm[8];
+    bool finite = true;
 #pragma unroll
     for (int i = 0; i < 8; ++i) xp[i] = xm[i] = 0.0;
     for (int node = 0; node < 3; ++node) {
@@ -906,6 +907,12 @@ __device__ inline V8<double> flux_dots(const Iface<double>& s, int sax, double g
         for (int i = 0; i < 8; ++i) g[i] = s.qm.c[i] + dots::ROOT[node] * (s.qp.c[i] - s.qm.c[i]);
         double A[8][8], V[8][8];
         dots::right_eigenvectors(sax, g, gamma, A);
+        // numpy's pinv raises (its SVD does not converge) on a matrix with a
+        // non-finite entry: the reference's run ends inside this step
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) finite = finite && ah_isfinite(A[r][c]);
         const dots::Speeds w = dots::speeds(g, g[5], gamma);
         const double vx = g[1];
         const double lam[8] = {ah_fabs(vx - w.cf),  ah_fabs(vx - w.cax), ah_fabs(vx - w.csl), ah_fabs(vx),
@@ -962,7 +969,7 @@ __device__ inline V8<double> flux_dots(const Iface<double>& s, int sax, double g
     V8<double> F = zero8<double, MASK>();
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-        if (VS<MASK>::live(i)) F.c[i] = 0.5 * (s.fp.c[i] + s.fm.c[i]) - 0.5 * (xp[i] - xm[i]);
+        if (VS<MASK>::live(i)) F.c[i] = finite ? 0.5 * (s.fp.c[i] + s.fm.c[i]) - 0.5 * (xp[i] - xm[i]) : NAN;
     return F;
 }
 

@@ -140,6 +140,13 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
         if (mask == 0x77) {
             if (solver == LLF) return by_recon<R, 0x77, LLF, 2, SPLIT>(recon, ks);
             if (solver == HLLD) return by_recon<R, 0x77, HLLD, 2, SPLIT>(recon, ks);
+            // HLLC on 2D MHD states: reachable from the reference's options
+            // (a magnetic-2D configuration keeps 'hllc'; generic.py:266-269
+            // resets it for 1D magnetic configurations only).  Osher-Solomon
+            // is not: its dissipation couples v_z and B_z into the update, and
+            // the reference's own 2D call raises (the reshape at
+            // solvers.py:281 assumes 1D arrays)
+            if (solver == HLLC) return by_recon<R, 0x77, HLLC, 2, SPLIT>(recon, ks);
         }
     }
     return false;

@@ -438,6 +438,19 @@ __device__ inline V8<R> riemann(const Iface<R>& s, R lam, int sax, R Bn, R gamma
         return sax == 0 ? flux_hlld<R, MASK, 0>(s, Bn, gamma) : flux_hlld<R, MASK, 1>(s, Bn, gamma);
 }
 
+// Osher-Solomon: a non-finite flux marks the step failed (flux_dots returns
+// NaN where the reference's pinv raises), which keeps the last good state
+template <int SOLVER, int MASK, class R>
+__device__ inline void dots_fail(const V8<R>& F, ProbState* ps) {
+    if constexpr (SOLVER == DOTS) {
+        bool nf = false;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (VS<MASK>::live(c)) nf = nf || !ah_isfinite(F.c[c]);
+        if (nf) atomicOr(&ps->fail_flag, 1);
+    }
+}
+
 // Block-wide max of a non-negative double, then one atomic per block.
 __device__ inline void block_max_atomic(double v, unsigned long long* dst, double* red) {
     // NaN-aware via bit patterns: reduce as unsigned long long
@@ -745,6 +758,7 @@ __global__ __launch_bounds__(256) void sweep1d_kernel(SweepArgs a) {
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = W(e(k)).c[5];
         V8<R> F = riemann<R, MASK, SOLVER, 0>(f, lam, sax, Bn, gR, 1, a.compat != 0, SPLIT);
+        dots_fail<SOLVER, MASK>(F, ps);
         if constexpr (RECON == PPM) {
             if (a.av) {
                 // artificial viscosity at interface k (schemes/ppm.py:138-170,
@@ -1406,6 +1420,7 @@ void flux2d_kernel(SweepArgs a) {
         }
         const R lam = LAM ? nmax(rad_s[ia][ik], rad_s[ia][ik + 1]) : R(0);
         const V8<R> F = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0, SPLIT);
+        dots_fail<SOLVER, MASK>(F, ps);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (V::live(c)) {
@@ -1451,6 +1466,7 @@ void flux2d_kernel(SweepArgs a) {
         R Bn = R(0);
         if constexpr (SOLVER == HLLD) Bn = bn_s[ia0][ik];
         const V8<R> Fc = riemann<R, MASK, SOLVER, AX>(f, lam, sax, Bn, gR, sax + 1, a.compat != 0, SPLIT);
+        dots_fail<SOLVER, MASK>(Fc, ps);
         const int k = s0 + ik;
 #pragma unroll
         for (int c = 0; c < 8; ++c)

@@ -419,6 +419,41 @@ def test_mhd_ragged_tiles():
     assert e <= 1e-12
 
 
+@pytest.mark.parametrize("config,subgrid,solver,steps", [("orszag-tang", "weno5", "hllc", 4),
+                                                          ("orszag-tang", "plm", "hllc", 4),
+                                                          ("mhd rotor", "ppm", "hllc", 3)])
+def test_mhd_2d_hllc(config, subgrid, solver, steps):
+    """2D MHD with constrained transport under HLLC (a magnetic-2D
+    configuration keeps 'hllc': generic.py:266-269 resets it for 1D magnetic
+    configurations only), 16^2 from the configuration's own initial state:
+    device vs oracle.  The corner speeds take the HLL estimate of the solver
+    category 'hll' (mag_field.py:135-149)."""
+    from astrea_amd import ic
+    from astrea_amd.options import handle_variables
+    from astrea_amd.plan import Plan
+    sv = handle_variables({"config": config, "cells": 16, "solver": solver, "subgrid": subgrid,
+                           "timestep": "ssprk(3,3)", "dimension": 2})
+    q0 = np.asarray(ic.initialise(sv))
+    n = q0.shape[0]
+    _say(f"[gpu] 2D MHD {config} {subgrid}/{solver} {n}^2, {steps} steps")
+    plan = Plan(2, subgrid, solver, "ssprk(3,3)", [dict(n0=n, n1=n, boundary=sv.boundary, dx=sv.dx, gamma=sv.gamma,
+                                                        cfl=sv.cfl, t_end=1e9, max_steps=steps)], varmask=0x77,
+                magnetic_2d=True, solver_category=sv.solver_category)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=steps)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver=solver, boundary=sv.boundary, gamma=sv.gamma, cfl=sv.cfl,
+                 dx=sv.dx, magnetic_2d=True)
+    with np.errstate(all="ignore"):
+        ref = O.run(q0, s, 1e9, max_steps=steps)
+    assert st["steps"] == ref.steps == steps and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
 @pytest.mark.parametrize("n,subgrid,solver", [(17, "weno5", "hlld"), (24, "ppm", "hlld"), (20, "plm", "llf")])
 def test_mhd_edge_boundary(n, subgrid, solver):
     """The constrained-transport path with edge (outlet) boundaries, whose
