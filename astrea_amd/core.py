@@ -15,7 +15,7 @@ import torch
 
 from . import ic, options
 from .output import SnapshotFile
-from .plan import Plan, infer_varmask
+from .plan import Plan, infer_varmask, solver_varmask
 
 
 def plan_for(sim_variables, grids, precision=None):
@@ -26,7 +26,8 @@ def plan_for(sim_variables, grids, precision=None):
         problems.append(dict(n0=g.shape[0], n1=g.shape[1] if dims == 2 else 1, boundary=sv.boundary, dx=sv.dx,
                              gamma=sv.gamma, cfl=sv.cfl, t_end=sv.t_end, checkpoints=sv.checkpoints,
                              max_steps=sv.max_steps))
-    mask = infer_varmask(grids, dims, sv.magnetic_2d)
+    mask = solver_varmask(infer_varmask(grids, dims, sv.magnetic_2d), dims,
+                          options.solver_of(sv.solver, sv.solver_category))
     return Plan(dims, sv.subgrid, sv.solver, sv.timestep, problems, limiter=sv.limiter, flattener=sv.flattener,
                 precision=precision or sv.precision_mode, varmask=mask, magnetic_2d=sv.magnetic_2d,
                 solver_category=sv.solver_category, av=sv.av, ppm_author=sv.ppm_author)

@@ -32,6 +32,18 @@ def infer_varmask(grids, dims, magnetic):
     raise NotImplementedError("2D states with nonzero v_z or B_z have no device path")
 
 
+def solver_varmask(mask, dims, solver):
+    """Widen an inferred component set to one the solver is instantiated for
+    (``solver`` an AH_* enum): HLLD and entropy-stable in 1D run on all eight
+    components, HLLD in 2D on the six of 2D MHD (on hydrodynamic data its
+    field components stay zero, as they do in the reference's arrays)."""
+    if dims == 1 and solver in (L.AH_HLLD, L.AH_ES):
+        return L.AH_VARS_ALL
+    if dims == 2 and solver == L.AH_HLLD and mask == L.AH_VARS_H2:
+        return L.AH_VARS_M6
+    return mask
+
+
 class Plan:
     def __init__(self, dims, subgrid, solver, timestep, problems, *, limiter="minmod", flattener="off",
                  precision="fp64", varmask=L.AH_VARS_H1, compat="astrea", magnetic_2d=False, device="cuda:0",

@@ -140,3 +140,13 @@ def test_no_kernel_is_instantiated_under_two_fp_modes():
                 where.setdefault(sym, set()).add(tuple(flags))
     mixed = sorted(s for s, f in where.items() if len(f) > 1)
     assert not mixed, mixed[:5]
+
+
+def test_solver_varmask_widens_for_the_solvers_built_on_more_components():
+    from astrea_amd.plan import solver_varmask
+    assert solver_varmask(L.AH_VARS_H1, 1, L.AH_HLLD) == L.AH_VARS_ALL
+    assert solver_varmask(L.AH_VARS_H1, 1, L.AH_ES) == L.AH_VARS_ALL
+    assert solver_varmask(L.AH_VARS_H1, 1, L.AH_HLLC) == L.AH_VARS_H1
+    assert solver_varmask(L.AH_VARS_H2, 2, L.AH_HLLD) == L.AH_VARS_M6
+    assert solver_varmask(L.AH_VARS_H2, 2, L.AH_DOTS) == L.AH_VARS_H2
+    assert solver_varmask(L.AH_VARS_M6, 2, L.AH_HLLD) == L.AH_VARS_M6
