@@ -147,6 +147,10 @@ inline bool select_kernels(int mask, int recon, int solver, KernelSet* ks) {
             // the reference's own 2D call raises (the reshape at
             // solvers.py:281 assumes 1D arrays)
             if (solver == HLLC) return by_recon<R, 0x77, HLLC, 2, SPLIT>(recon, ks);
+            // Lax-Wendroff with the sound speed sqrt(gamma p / rho), as in 1D MHD
+            if constexpr (!SPLIT) {
+                if (solver == LW) return by_recon<R, 0x77, LW, 2, SPLIT>(recon, ks);
+            }
         }
     }
     return false;

@@ -59,7 +59,7 @@ enum ah_ppm_author { AH_PPM_MC = 0, AH_PPM_C = 1, AH_PPM_PH = 2 };
 /* solvers.py:69-75 (LLF), 92-138 (HLLC), 142-232 (HLLD), 236-284 (Osher-Solomon
    with the Dumbser-Toro path integral; FP64, 1D hydro/MHD and 2D hydro),
    288-391 (entropy-stable as the reference writes it; FP64, 1D, all 8 variables),
-   79-88 (Lax-Wendroff, FP64, hydrodynamic 1D/2D: the coefficient ss^2/max|lambda|
+   79-88 (Lax-Wendroff, FP64, 1D/2D hydrodynamics and MHD: the coefficient ss^2/max|lambda|
    with ss the sound speed sqrt(gamma p/rho); the reference takes ss from a
    LAPACK-order-dependent eigenvalue column instead, so its results are not
    reproduced) */

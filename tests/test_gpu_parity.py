@@ -421,13 +421,16 @@ def test_mhd_ragged_tiles():
 
 @pytest.mark.parametrize("config,subgrid,solver,steps", [("orszag-tang", "weno5", "hllc", 4),
                                                           ("orszag-tang", "plm", "hllc", 4),
-                                                          ("mhd rotor", "ppm", "hllc", 3)])
-def test_mhd_2d_hllc(config, subgrid, solver, steps):
+                                                          ("mhd rotor", "ppm", "hllc", 3),
+                                                          ("orszag-tang", "plm", "lw", 4),
+                                                          ("mhd rotor", "weno5", "lw", 3)])
+def test_mhd_2d_hllc_and_lw(config, subgrid, solver, steps):
     """2D MHD with constrained transport under HLLC (a magnetic-2D
     configuration keeps 'hllc': generic.py:266-269 resets it for 1D magnetic
-    configurations only), 16^2 from the configuration's own initial state:
-    device vs oracle.  The corner speeds take the HLL estimate of the solver
-    category 'hll' (mag_field.py:135-149)."""
+    configurations only) and Lax-Wendroff (sound-speed form), 16^2 from the
+    configuration's own initial state: device vs oracle.  The corner speeds
+    follow the solver category (mag_field.py:135-159): the HLL estimate for
+    HLLC, the Jacobian's extreme eigenvalues for Lax-Wendroff."""
     from astrea_amd import ic
     from astrea_amd.options import handle_variables
     from astrea_amd.plan import Plan
