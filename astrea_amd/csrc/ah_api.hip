@@ -239,8 +239,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     // artificial viscosity: the reference computes it for PPM only (ppm.py:94-95),
     // and its 2D form cannot run (ppm.py:149-154 adds arrays of different shapes)
     if (sch->av && (sch->dims != 1 || sch->recon != AH_PPM)) return AH_ERR_UNSUPPORTED;
-    if (sch->magnetic_2d && (sch->dims != 2 || !(sch->varmask & 0x60) || sch->rk == AH_SSPRK104 ||
-                             sch->flattener != AH_FLAT_OFF))
+    if (sch->magnetic_2d && (sch->dims != 2 || !(sch->varmask & 0x60) || sch->flattener != AH_FLAT_OFF))
         return AH_ERR_UNSUPPORTED;
     ah_plan* p = new (std::nothrow) ah_plan();
     if (!p) return AH_ERR_NOMEM;
@@ -300,7 +299,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
         case AH_RK4: p->nregs = 7; break;
         case AH_SSPRK53: p->nregs = 6; break;
         case AH_SSPRK54: p->nregs = 5; break;
-        case AH_SSPRK104: p->nregs = 3; break;
+        case AH_SSPRK104: p->nregs = sch->magnetic_2d ? 4 : 3; break;  // + the refined increment
     }
     // problem tables and tiles
     std::vector<ProbDesc> pd(nprob);
@@ -454,7 +453,7 @@ int ah_plan_bind(ah_plan* p, double* state, void* ws, size_t ws_bytes) {
         case AH_RK4: nk = 3; nl = 3; break;
         case AH_SSPRK53: nk = 4; nl = 2; break;
         case AH_SSPRK54: nk = 4; nl = 1; break;
-        case AH_SSPRK104: nk = 3; break;
+        case AH_SSPRK104: nk = p->sch.magnetic_2d ? 4 : 3; break;
     }
     for (int r = 0; r < nk; ++r) {
         p->regs[1 + r] = (double*)c;
@@ -614,13 +613,53 @@ static void launch_combine(ah_plan* p, int op, hipStream_t s, bool stored_dt = f
 // register each stage combination writes (for the MHD refinement)
 static int target_reg(int op) {
     switch (op) {
-        case OP_EULER: case OP_22_1: case OP_33_2: case OP_43_3: case OP_RK4_3: case OP_53_4: case OP_54_4: return 0;
+        case OP_EULER: case OP_22_1: case OP_33_2: case OP_43_3: case OP_RK4_3: case OP_53_4: case OP_54_4:
+        case OP_104_FINAL: return 0;
         case OP_22_0: case OP_33_0: case OP_43_0: case OP_RK4_0: case OP_53_0: case OP_54_0: return 1;
         case OP_33_1: case OP_43_1: case OP_RK4_1: case OP_53_1: case OP_54_1: return 2;
-        case OP_43_2: case OP_RK4_2: case OP_53_2: case OP_54_2: return 3;
+        case OP_43_2: case OP_RK4_2: case OP_53_2: case OP_54_2: case OP_104_K5: return 3;
         case OP_53_3: case OP_54_3: return 4;
     }
     return -1;
+}
+
+// SSPRK(10,4) stages that add a refined increment: the addition that
+// follows the refinement of K4 = dt/6 L, or -1
+static int increment_add(int op) {
+    switch (op) {
+        case OP_104_FIRST: return OP_104_ADD_U;
+        case OP_104_ACC1: return OP_104_ADD_K1;
+        case OP_104_K2FIRST: return OP_104_ADD_K3;
+        case OP_104_ACC2: return OP_104_ADD_K2;
+    }
+    return -1;
+}
+
+static MhdArgs mhd_args(ah_plan* p);
+
+// A stage's combination followed, with constrained transport, by refine_grid
+// (evolvers.py:63-67) of the register it wrote -- or, for the SSPRK(10,4)
+// accumulations, of the increment before it is added
+static void combine_refine(ah_plan* p, int op, hipStream_t s, bool stored_dt = false) {
+    if (!p->sch.magnetic_2d) {
+        launch_combine(p, op, s, stored_dt);
+        return;
+    }
+    MhdArgs m = mhd_args(p);
+    const int add = increment_add(op);
+    if (add >= 0) {
+        launch_combine(p, OP_104_INC, s, stored_dt);
+        m.K = p->regs[4];
+        p->ks.refine(m, blocks_for(p, p->tl[7]), s);
+        p->ks.copyB(m, p->regs[4], blocks_for(p, p->tl[5]), s);
+        launch_combine(p, add, s, stored_dt);
+        return;
+    }
+    launch_combine(p, op, s, stored_dt);
+    double* K = p->regs[target_reg(op)];
+    m.K = K;
+    p->ks.refine(m, blocks_for(p, p->tl[7]), s);
+    p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
 }
 
 static MhdArgs mhd_args(ah_plan* p) {
@@ -688,13 +727,7 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
         if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[7]), s);
-        if (!fuse33 && !(k1_live && S.op == OP_33_0)) launch_combine(p, S.op, s);
-        if (mhd) {
-            double* K = p->regs[target_reg(S.op)];
-            m.K = K;
-            p->ks.refine(m, blocks_for(p, p->tl[7]), s);
-            p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
-        }
+        if (!fuse33 && !(k1_live && S.op == OP_33_0)) combine_refine(p, S.op, s);
     }
     {
         ProfScope ps_(p, AH_K_ADVANCE, s);
@@ -1123,15 +1156,8 @@ int ah_evolve_time(ah_plan* p, int prob, int parity, double* grid, const double*
                 p->ks.magL(m, blocks_for(p, p->tl[5]), s);
             }
         }
-        launch_combine(p, S.op, s, true);
-        if (mhd) {
-            // refine_grid after every stage (evolvers.py:63-67)
-            MhdArgs m = mhd_args(p);
-            double* K = p->regs[target_reg(S.op)];
-            m.K = K;
-            p->ks.refine(m, blocks_for(p, p->tl[7]), s);
-            p->ks.copyB(m, K, blocks_for(p, p->tl[5]), s);
-        }
+        // refine_grid after every stage (evolvers.py:63-67)
+        combine_refine(p, S.op, s, true);
     }
     p->U = U_bound;
     p->regs[0] = U_bound;

@@ -127,7 +127,10 @@ enum RkOp {
     OP_RK4_0, OP_RK4_1, OP_RK4_2, OP_RK4_3,
     OP_53_0, OP_53_1, OP_53_2, OP_53_3, OP_53_4,
     OP_54_0, OP_54_1, OP_54_2, OP_54_3, OP_54_4,
-    OP_104_FIRST, OP_104_ACC1, OP_104_K5, OP_104_K2FIRST, OP_104_ACC2, OP_104_FINAL
+    OP_104_FIRST, OP_104_ACC1, OP_104_K5, OP_104_K2FIRST, OP_104_ACC2, OP_104_FINAL,
+    // SSPRK(10,4) with constrained transport refines the increment before
+    // adding it (evolvers.py:89-90, 97-99): K4 := dt/6 L, refined, then added
+    OP_104_INC, OP_104_ADD_U, OP_104_ADD_K1, OP_104_ADD_K3, OP_104_ADD_K2
 };
 
 struct CombineArgs {
@@ -213,6 +216,11 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
                 case OP_104_K5: K3[j] = 3.0 / 5.0 * U[j] + 6.0 / 15.0 * K1[j] + 1.0 / 15.0 * dt * L; break;
                 case OP_104_K2FIRST: K2[j] = K3[j] + (1.0 / 6.0) * dt * L; break;
                 case OP_104_ACC2: K2[j] = K2[j] + (1.0 / 6.0) * dt * L; break;
+                case OP_104_INC: K4[j] = (1.0 / 6.0) * dt * L; break;
+                case OP_104_ADD_U: K1[j] = U[j] + K4[j]; break;
+                case OP_104_ADD_K1: K1[j] = K1[j] + K4[j]; break;
+                case OP_104_ADD_K3: K2[j] = K3[j] + K4[j]; break;
+                case OP_104_ADD_K2: K2[j] = K2[j] + K4[j]; break;
                 case OP_104_FINAL:
                     U[j] = -11.0 / 35.0 * U[j] + 5.0 / 7.0 * K3[j] + 3.0 / 5.0 * K2[j] + 1.0 / 10.0 * dt * L;
                     break;

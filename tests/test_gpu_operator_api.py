@@ -164,6 +164,7 @@ API_CASES = [
     (2, "plm", "llf", "edge", False, "ssprk(5,4)"),
     (2, "weno5", "hlld", "wrap", True, "ssprk(3,3)"),
     (2, "plm", "llf", "edge", True, "rk4"),
+    (2, "weno5", "hlld", "wrap", True, "ssprk(10,4)"),
 ]
 
 

@@ -457,6 +457,35 @@ def test_mhd_2d_hllc_and_lw(config, subgrid, solver, steps):
     assert e <= 1e-12
 
 
+@pytest.mark.parametrize("subgrid,solver,bc", [("weno5", "hlld", "wrap"), ("plm", "llf", "edge")])
+def test_mhd_ssprk104_trajectory(subgrid, solver, bc):
+    """SSPRK(10,4) with constrained transport: its accumulating stages add a
+    refined increment, k += refine_grid(dt/6 L) (evolvers.py:89-90, 97-99),
+    the others refine the combined register; Orszag-Tang 16^2, 3 steps,
+    device vs oracle."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    n = 16
+    q0 = ic.initial_average(torch.from_numpy(ic.orszag_tang(n)), 5 / 3, bc, subgrid == "weno5").numpy()
+    _say(f"[gpu] mhd ssprk(10,4) {n}^2 {subgrid}/{solver}/{bc}")
+    plan = Plan(2, subgrid, solver, "ssprk(10,4)", [dict(n0=n, n1=n, boundary=bc, dx=1 / n, gamma=5 / 3, cfl=.4,
+                                                         t_end=1e9, max_steps=3)], varmask=0x77, magnetic_2d=True)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(chunk=3)[0]
+    out = plan.unpack(0)
+    plan.close()
+    s = O.Scheme(dimension=2, subgrid=subgrid, solver=solver, timestep="ssprk(10,4)", boundary=bc, gamma=5 / 3,
+                 cfl=.4, dx=1 / n, magnetic_2d=True)
+    with np.errstate(all="ignore"):
+        ref = O.run(q0, s, 1e9, max_steps=3)
+    assert st["steps"] == ref.steps == 3 and not st["failed"]
+    e = np.max(rel_l1(out, ref.grid))
+    _say(f"[gpu]   rel L1 vs oracle {e:.2e}")
+    assert e <= 1e-12
+
+
 @pytest.mark.parametrize("n,subgrid,solver", [(17, "weno5", "hlld"), (24, "ppm", "hlld"), (20, "plm", "llf")])
 def test_mhd_edge_boundary(n, subgrid, solver):
     """The constrained-transport path with edge (outlet) boundaries, whose
