@@ -633,3 +633,33 @@ def test_fused_sweep_refused_where_not_built():
         p1 = dict(prob, n1=1) if dims == 1 else prob
         with pytest.raises(L.AstreaHipError, match="not implemented"):
             Plan(dims, sg, sol, "ssprk(3,3)", [p1], varmask=vm, fuse2d=True)
+
+
+# ---- the option surface ---------------------------------------------------------
+def test_every_reachable_option_combination_creates_a_plan():
+    """Every configuration category (one configuration each, 1D and 2D,
+    hydrodynamic and magnetic) x subgrid x solver x time stepper of the
+    reference's option surface creates a device plan through core.plan_for,
+    except the combinations the reference itself cannot run: Osher-Solomon on
+    2D MHD and entropy-stable in 2D, whose reshapes (solvers.py:281,389)
+    assume 1D arrays."""
+    from astrea_amd import _lib as L, core, ic
+    from astrea_amd.options import handle_variables
+    configs = {"sod": 1, "sedov": 1, "shu-osher": 1, "slow": 1, "sin": 1, "square": 1, "brio-wu": 1,
+               "lin mhd": 1, "orszag-tang": 2, "kelvin-helmholtz": 2, "ll3": 2, "ivc": 2, "mhd rotor": 2}
+    steps = ["euler", "rk4", "ssprk(2,2)", "ssprk(3,3)", "ssprk(4,3)", "ssprk(5,3)", "ssprk(5,4)", "ssprk(10,4)"]
+    refused = set()
+    _say("[gpu] plan creation over the option surface")
+    for cfg, dims in configs.items():
+        for so in ("lf", "lw", "hllc", "hlld", "os", "es"):
+            for sg in ("pcm", "plm", "ppm", "weno3", "weno5", "weno7"):
+                for ts in steps:
+                    sv = handle_variables({"config": cfg, "cells": 16, "solver": so, "subgrid": sg, "timestep": ts,
+                                           "dimension": dims})
+                    g = np.asarray(ic.initialise(sv))
+                    try:
+                        core.plan_for(sv, [g]).close()
+                    except L.AstreaHipError:
+                        refused.add((cfg, so))
+    expect = {(c, "es") for c, d in configs.items() if d == 2} | {("orszag-tang", "os"), ("mhd rotor", "os")}
+    assert refused == expect
