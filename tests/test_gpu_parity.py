@@ -1209,13 +1209,14 @@ SWEEP_STEPS = ["euler", "rk4", "ssprk(2,2)", "ssprk(3,3)", "ssprk(4,3)", "ssprk(
 
 def _sweep_cases():
     """Every configuration category x solver the reference can run, each with
-    a subgrid and a time stepper taken in rotation."""
+    two (subgrid, time stepper) pairs taken in rotation."""
     cases, i = [], 0
     for c, (cfg, dims) in enumerate(SWEEP_CONFIGS.items()):
         for so in ("lf", "lw", "hllc", "hlld", "os", "es"):
             if dims == 2 and (so == "es" or (so == "os" and cfg in ("orszag-tang", "mhd rotor"))):
                 continue
-            cases.append((cfg, dims, so, SWEEP_SUBGRIDS[(i + c) % 6], SWEEP_STEPS[i % 8]))
+            for r in (0, 3):
+                cases.append((cfg, dims, so, SWEEP_SUBGRIDS[(i + c + r) % 6], SWEEP_STEPS[(i + r + 1) % 8]))
             i += 1
     return cases
 
@@ -1227,7 +1228,8 @@ def _floored_l1(a, b, floor=1e-4):
     rounding noise of a different pattern in each implementation."""
     ax = tuple(range(a.ndim - 1))
     num, den = np.abs(a - b).sum(ax), np.abs(b).sum(ax)
-    return float(np.max(num / np.maximum(den, floor * den.max())))
+    den = np.maximum(den, floor * den.max())
+    return float(np.max(num / np.where(den > 0, den, 1.0)))
 
 
 @pytest.mark.parametrize("case", _sweep_cases(), ids=lambda c: "-".join(map(str, c)).replace(" ", "_"))
@@ -1249,27 +1251,33 @@ def test_option_surface_parity(case):
                  timestep=sv.timestep, boundary=sv.boundary, gamma=sv.gamma, cfl=sv.cfl, dx=sv.dx,
                  magnetic_2d=sv.magnetic_2d)
     q0 = ic.initialise(sv)
-    refs = []
+    # the step the reference raises in: the oracle with LAPACK eigenvalues
+    # raises where the reference does (a non-finite Jacobian state, or the
+    # Osher-Solomon pseudo-inverse of a non-finite matrix)
+    bad = None
     for k in (1, 2):
         try:
             with np.errstate(all="ignore"):
-                r = O.run(q0, s, sv.t_end, sv.checkpoints, k)
+                O.run(q0, replace(s, eig="lapack"), sv.t_end, sv.checkpoints, k)
         except np.linalg.LinAlgError:
-            r = None
-        refs.append(r)
-        if r is None or not np.isfinite(r.grid).all() or r.steps < k:
+            bad = k
             break
-    last = refs[-1]
-    if last is None or not np.isfinite(last.grid).all():
-        bad = len(refs)
-        _say(f"[gpu]   oracle fails in step {bad}; device steps {st['steps']} failed {st['failed']}")
+    with np.errstate(all="ignore"):
+        ref = O.run(q0, s, sv.t_end, sv.checkpoints, 2 if bad is None else bad - 1) if bad != 1 else None
+    if bad is not None:
+        _say(f"[gpu]   the reference raises in step {bad}; device steps {st['steps']} failed {st['failed']}")
         assert st["failed"] and st["steps"] == bad - 1
-        good = q0 if bad == 1 else refs[bad - 2].grid
-        assert _floored_l1(out, good) <= tol
+        good = q0 if ref is None else ref.grid
+        # the state before the failing step, non-finite entries included
+        # (a one-stage scheme updates with a non-finite flux and raises in
+        # the next step's eigenvalue call)
+        fin = np.isfinite(good)
+        assert np.array_equal(fin, np.isfinite(out))
+        assert _floored_l1(np.where(fin, out, 0), np.where(fin, good, 0)) <= tol
         return
-    assert not st["failed"] and st["steps"] == last.steps
-    e = _floored_l1(out, last.grid)
-    _say(f"[gpu]   {last.steps} steps, L1 vs oracle {e:.1e}")
+    assert not st["failed"] and st["steps"] == ref.steps
+    e = _floored_l1(out, ref.grid)
+    _say(f"[gpu]   {ref.steps} steps, L1 vs oracle {e:.1e}")
     assert e <= tol
 
 
