@@ -101,6 +101,7 @@ struct ah_plan {
     // device tables
     ProbDesc* d_pd = nullptr;
     ProbState* d_ps = nullptr;
+    unsigned* d_done = nullptr;  // per problem: tiles of the step's last combination done
     // tile lists: 0 = 1D sweep, 1/2 = recon ax0/ax1, 3/4 = flux ax0/ax1, 5 = cells (combine),
     // 6/7 = transverse reconstruction ax0/ax1 (MHD)
     TileList tl[8];
@@ -356,7 +357,9 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     bool up = hipMalloc((void**)&p->d_pd, nprob * sizeof(ProbDesc)) == hipSuccess &&
               hipMemcpy(p->d_pd, pd.data(), nprob * sizeof(ProbDesc), hipMemcpyHostToDevice) == hipSuccess &&
               hipMalloc((void**)&p->d_ps, nprob * sizeof(ProbState)) == hipSuccess &&
-              hipMemset(p->d_ps, 0, nprob * sizeof(ProbState)) == hipSuccess;
+              hipMemset(p->d_ps, 0, nprob * sizeof(ProbState)) == hipSuccess &&
+              hipMalloc((void**)&p->d_done, nprob * sizeof(unsigned)) == hipSuccess &&
+              hipMemset(p->d_done, 0, nprob * sizeof(unsigned)) == hipSuccess;
     for (TileList& L : p->tl) {
         if (!up) break;
         L.count = (int)L.full.size();
@@ -389,7 +392,7 @@ int ah_plan_destroy(ah_plan* p) {
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     int rc = AH_OK;
     if (p->graph_exec && hipGraphExecDestroy(p->graph_exec) != hipSuccess) rc = AH_ERR_HIP;
-    void* bufs[] = {p->d_pd, p->d_ps};
+    void* bufs[] = {p->d_pd, p->d_ps, p->d_done};
     for (void* b : bufs)
         if (b && hipFree(b) != hipSuccess) rc = AH_ERR_HIP;
     for (TileList& L : p->tl) {
@@ -593,8 +596,11 @@ static void launch_space(ah_plan* p, const double* X, double* Lout, int stage0, 
 
 // stored_dt: combine with the dt stored in the time state (constrained
 // transport stores it after stage 0; ah_evolve_time is handed it)
-static void launch_combine(ah_plan* p, int op, hipStream_t s, bool stored_dt = false) {
+static void launch_combine(ah_plan* p, int op, hipStream_t s, bool stored_dt = false, bool finish = false) {
     CombineArgs c;
+    c.finish = finish ? 1 : 0;
+    c.ct = p->sch.dims == 1 ? 256 : 1024;
+    c.done = p->d_done;
     c.live_dt = (p->sch.magnetic_2d || stored_dt) ? 0 : 1;
     c.dims = p->sch.dims;
     for (int r = 0; r < 9; ++r) c.regs[r] = p->regs[r];
@@ -640,9 +646,9 @@ static MhdArgs mhd_args(ah_plan* p);
 // A stage's combination followed, with constrained transport, by refine_grid
 // (evolvers.py:63-67) of the register it wrote -- or, for the SSPRK(10,4)
 // accumulations, of the increment before it is added
-static void combine_refine(ah_plan* p, int op, hipStream_t s, bool stored_dt = false) {
+static void combine_refine(ah_plan* p, int op, hipStream_t s, bool stored_dt = false, bool finish = false) {
     if (!p->sch.magnetic_2d) {
-        launch_combine(p, op, s, stored_dt);
+        launch_combine(p, op, s, stored_dt, finish);
         return;
     }
     MhdArgs m = mhd_args(p);
@@ -727,14 +733,14 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
         if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[7]), s);
-        if (!fuse33 && !(k1_live && S.op == OP_33_0)) combine_refine(p, S.op, s);
+        // without constrained transport the step's last combination also ends
+        // the step (finish_problem in the block that combines a problem's last tile)
+        const bool last = st + 1 == p->prog.size() && !mhd;
+        if (!fuse33 && !(k1_live && S.op == OP_33_0)) combine_refine(p, S.op, s, false, last);
     }
-    {
+    if (mhd) {
         ProfScope ps_(p, AH_K_ADVANCE, s);
-        if (mhd)
-            hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
-        else
-            hipLaunchKernelGGL(finish_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
+        hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
     }
     if (p->cap.ctl) {
         snapshot_any(p, p->cap.prob, nullptr, s, p->cap);

@@ -61,13 +61,8 @@ __global__ void advance_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
 // updates of dt_kernel and advance_kernel in one launch, with the same
 // outcome -- a failure seen at stage 0 (non-finite radii) leaves dt and the
 // checkpoint index as they were, a later one records this step's dt.
-__global__ void finish_kernel(const ProbDesc* pd, ProbState* ps, int nprob, int dims) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nprob) return;
-    ProbState& s = ps[p];
-    if (!s.active) return;
+__device__ inline void finish_problem(const ProbDesc& d, ProbState& s, int dims) {
     s.aflags = 0ull;
-    const ProbDesc d = pd[p];
     const double e0 = __longlong_as_double((long long)s.eig[0]);
     const double e1 = dims == 2 ? __longlong_as_double((long long)s.eig[1]) : e0;
     if (!(e0 <= 1.7976931348623157e308) || !(e1 <= 1.7976931348623157e308)) {
@@ -93,6 +88,14 @@ __global__ void finish_kernel(const ProbDesc* pd, ProbState* ps, int nprob, int 
     const bool more_time = (s.t <= d.t_end) && !(s.t == d.t_end);
     const bool more_steps = d.max_steps < 0 || s.steps < d.max_steps;
     s.active = more_time && more_steps;
+}
+
+__global__ void finish_kernel(const ProbDesc* pd, ProbState* ps, int nprob, int dims) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob) return;
+    ProbState& s = ps[p];
+    if (!s.active) return;
+    finish_problem(pd[p], s, dims);
 }
 
 __global__ void reset_kernel(const ProbDesc* pd, ProbState* ps, int nprob) {
@@ -137,7 +140,7 @@ struct CombineArgs {
     double* regs[9];
     const double* L;
     const ProbDesc* pd;
-    const ProbState* ps;
+    ProbState* ps;
     const Tile* tiles;       // cell tiles: (problem, first cell, cell count)
     const int* ntiles_dev;
     long long plane;
@@ -145,6 +148,13 @@ struct CombineArgs {
     int op;
     int live_dt;   // 1: dt from step_dt (no dt launch in this step program), 0: the stored dt
     int dims;
+    // finish: this combination ends the step, and the block that combines a
+    // problem's last tile also ends the problem's step (finish_problem); done
+    // counts each problem's combined tiles (reset by that block), ct = cells
+    // per tile
+    int finish;
+    int ct;
+    unsigned* done;
 };
 
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
@@ -153,7 +163,8 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const Tile tl = ld_const(a.tiles + ti);
         const ProbState& s = a.ps[tl.prob];
-        if (!ld_const(&s.active) || ld_const(&s.fail_flag)) continue;
+        if (!ld_const(&s.active)) continue;
+        if (!ld_const(&s.fail_flag)) {
         double dt;
         if (a.live_dt) {
             int hit;
@@ -224,6 +235,25 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a) {
                 case OP_104_FINAL:
                     U[j] = -11.0 / 35.0 * U[j] + 5.0 / 7.0 * K3[j] + 3.0 / 5.0 * K2[j] + 1.0 / 10.0 * dt * L;
                     break;
+            }
+        }
+        }
+        if (a.finish) {
+            // every block that combines one of the problem's tiles counts it
+            // once its threads have read the time state and written their
+            // cells; the block that counts the last one ends the problem's
+            // step.  No other block reads this problem's time state after
+            // counting, so no fence orders the update against them; the next
+            // launch sees it.
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const ProbDesc d = ld_const(&a.pd[tl.prob]);
+                const long long cells = (long long)d.n0 * d.n1;
+                const unsigned nt = (unsigned)((cells + a.ct - 1) / a.ct);
+                if (atomicAdd(&a.done[tl.prob], 1u) == nt - 1) {
+                    a.done[tl.prob] = 0u;
+                    finish_problem(d, a.ps[tl.prob], a.dims);
+                }
             }
         }
     }
