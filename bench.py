@@ -28,6 +28,7 @@ Prints one JSON line.  Only one GPU is used: --gpus N with N > 1 exits.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -54,10 +55,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=10)
-    ap.add_argument("--config", default="shear", choices=["shear", "ensemble", "serve", "copy", "mhd"],
+    ap.add_argument("--capacity-steps", type=int, default=3, help="capacity: timed steps")
+    ap.add_argument("--config", default="shear", choices=["shear", "ensemble", "serve", "copy", "mhd", "capacity"],
                     help="shear: 2D shear layer (default); ensemble: 512 ragged 1D Riemann problems; "
                          "serve: Poisson stream of 2D four-quadrant solves; copy: device copy of --bytes "
-                         "(calibrates the HBM traffic counters); mhd: 2D Orszag-Tang with constrained transport")
+                         "(calibrates the HBM traffic counters); mhd: 2D Orszag-Tang with constrained transport; "
+                         "capacity: the largest 2D grid one GPU holds")
     ap.add_argument("--mhd-n", type=int, default=2048, help="mhd: grid cells per axis")
     ap.add_argument("--mhd-steps", type=int, default=12, help="mhd: timed steps (the scheme fails after 16-18)")
     ap.add_argument("--mhd-paths", default="fp64,split", help="mhd: device paths (fp64, split)")
@@ -622,6 +625,64 @@ def bench_mhd(args):
     }
 
 
+def bench_capacity(args):
+    """The largest single 2D grid one GPU holds, per sweep form: a
+    four-quadrant Riemann problem (Lax-Liu configuration 3 states, built on
+    the device), PLM/HLLC (physical axis)/SSPRK(3,3), edge boundaries, its
+    side the largest multiple of 1024 whose state and workspace fit 90 % of
+    the free HBM (and under 2^31 cells, the plans' index range); a few
+    graph-replayed steps timed.  The one-launch form keeps no interface-state
+    planes, so it holds the larger grid."""
+    import torch
+    from astrea_amd.plan import Plan
+    # Lax-Liu 3: (rho, u, v, p) for x <= .5 / > .5 and y < .5 / >= .5
+    states = [(.138, 1.206, 1.206, .029), (.5323, 0., 1.206, .3), (.5323, 1.206, 0., .3), (1.5, 0., 0., 1.5)]
+    out = {}
+    for fuse in (True, False):
+        torch.cuda.empty_cache()
+        free, total = torch.cuda.mem_get_info()
+        probe = Plan(2, "plm", "hllc", "ssprk(3,3)", [dict(n0=256, n1=256, boundary="edge", dx=1 / 256, gamma=1.4,
+                                                          cfl=.4, t_end=1.)], varmask=0x17, compat="physical",
+                     fuse2d=fuse)
+        bpc = probe.bytes_per_cell()
+        probe.close()
+        n = int(math.sqrt(.9 * free / bpc)) // 1024 * 1024
+        while n * n >= 2 ** 31:
+            n -= 1024
+        steps = args.capacity_steps
+        _progress(f"[capacity] fused={fuse}: {bpc:.0f} B/cell, free {free / 2**30:.0f} GiB -> {n}^2")
+        plan = Plan(2, "plm", "hllc", "ssprk(3,3)", [dict(n0=n, n1=n, boundary="edge", dx=1 / n, gamma=1.4, cfl=.4,
+                                                          t_end=1e9, max_steps=steps + 2)], varmask=0x17,
+                    compat="physical", fuse2d=fuse)
+        plan.init_quadrants(0, states)
+        plan.reset()
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            plan.step_graph(1, 2)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            plan.step_graph(1, steps)
+            stream.synchronize()
+            secs = time.perf_counter() - t0
+        st = plan.status()[0]
+        assert st["steps"] == steps + 2 and not st["failed"], st
+        used = bpc * n * n
+        plan.close()
+        del plan
+        out["fused" if fuse else "two_launch"] = {
+            "n": n, "cells": n * n, "bytes_per_cell": bpc, "device_GiB": used / 2 ** 30,
+            "hbm_total_GiB": total / 2 ** 30, "ms_per_step": secs / steps * 1e3, "zu_per_s": n * n * steps / secs}
+        _progress(f"[capacity]   {n}^2: {secs / steps * 1e3:.0f} ms/step, {n * n * steps / secs:.3e} ZU/s")
+    best = max(out.values(), key=lambda r: r["cells"])
+    return {"metric": METRIC, "value": best["zu_per_s"], "unit": "zone-updates/s", "n_gpus": 1,
+            "steps": args.capacity_steps, "warmup": 2, "ms_per_step": best["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "largest 2D four-quadrant grid on one GPU, Lax-Liu 3 states, PLM/HLLC (physical "
+                                   "axis)/SSPRK(3,3), CFL 0.4, edge", "cells": best["cells"]},
+            "forms": out}
+
+
 def bench_copy(args):
     """Elementwise kernel over a known byte count (dst = src * 1, FP64): reads
     and writes exactly --bytes each.  Its FETCH_SIZE / WRITE_SIZE counters per
@@ -658,6 +719,9 @@ def main():
         return
     if args.config == "mhd":
         print(json.dumps(bench_mhd(args)))
+        return
+    if args.config == "capacity":
+        print(json.dumps(bench_capacity(args)))
         return
     if args.config != "shear":
         line = bench_ensemble(args) if args.config == "ensemble" else bench_serve(args)
