@@ -102,6 +102,7 @@ struct ah_plan {
     ProbDesc* d_pd = nullptr;
     ProbState* d_ps = nullptr;
     unsigned* d_done = nullptr;  // per problem: tiles of the step's last combination done
+    bool fold_finish = false;    // the last combination ends the step (few tiles per problem)
     // tile lists: 0 = 1D sweep, 1/2 = recon ax0/ax1, 3/4 = flux ax0/ax1, 5 = cells (combine),
     // 6/7 = transverse reconstruction ax0/ax1 (MHD)
     TileList tl[8];
@@ -304,6 +305,7 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
     }
     // problem tables and tiles
     std::vector<ProbDesc> pd(nprob);
+    long long max_ctiles = 0;
     long long total = 0;
     for (int i = 0; i < nprob; ++i) {
         const ah_problem& q = probs[i];
@@ -347,8 +349,14 @@ int ah_plan_create(const ah_scheme* sch, int nprob, const ah_problem* probs, ah_
         const int ct = sch->dims == 1 ? 256 : 1024;
         for (long long c0 = 0; c0 < cells; c0 += ct)
             p->tl[5].full.push_back({i, (int)c0, (int)std::min<long long>(ct, cells - c0), 0});
+        max_ctiles = std::max(max_ctiles, (cells + ct - 1) / ct);
     }
     p->total = total;
+    // each combination tile of the step's last combination counts itself on
+    // its problem's counter; beyond a few dozen tiles per problem the atomics
+    // on one address cost more than the separate end-of-step launch (1024^2:
+    // 1024 tiles, +5 us against the launch's 5 us)
+    p->fold_finish = !sch->magnetic_2d && max_ctiles <= 64;
     if (total >= (1ll << 31)) {
         delete p;
         return AH_ERR_INVALID;
@@ -733,14 +741,18 @@ static void launch_step(ah_plan* p, hipStream_t s) {
             hipLaunchKernelGGL(dt_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
         }
         if (mhd && st == 0) p->ks.bface(m, blocks_for(p, p->tl[7]), s);
-        // without constrained transport the step's last combination also ends
-        // the step (finish_problem in the block that combines a problem's last tile)
-        const bool last = st + 1 == p->prog.size() && !mhd;
+        // without constrained transport and with few combination tiles per
+        // problem the step's last combination also ends the step
+        // (finish_problem in the block that combines a problem's last tile)
+        const bool last = st + 1 == p->prog.size() && p->fold_finish;
         if (!fuse33 && !(k1_live && S.op == OP_33_0)) combine_refine(p, S.op, s, false, last);
     }
     if (mhd) {
         ProfScope ps_(p, AH_K_ADVANCE, s);
         hipLaunchKernelGGL(advance_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob);
+    } else if (!p->fold_finish) {
+        ProfScope ps_(p, AH_K_ADVANCE, s);
+        hipLaunchKernelGGL(finish_kernel, dim3(pb), dim3(256), 0, s, p->d_pd, p->d_ps, p->nprob, p->sch.dims);
     }
     if (p->cap.ctl) {
         snapshot_any(p, p->cap.prob, nullptr, s, p->cap);

@@ -330,7 +330,10 @@ int ah_evolve_space_static(const ah_scheme* scheme, const ah_problem* problem, i
  * returns the summed milliseconds and launch counts per kernel kind and
  * clears the record.  Without constrained transport a step has no dt launch
  * (its kernels evaluate dt) and AH_K_ADVANCE times the end-of-step launch
- * that stores dt and advances t; the middle SSPRK(3,3) stage has no
+ * that stores dt and advances t, which runs only on plans with more than 64
+ * combination tiles (1024 cells each in 2D, 256 in 1D) per problem; on the
+ * others the step's last combination launch does that work.  The middle
+ * SSPRK(3,3) stage has no
  * combination launch (the launch that finishes its L writes K2). */
 enum ah_kernel_kind {
     AH_K_SWEEP1D = 0, AH_K_RECON0 = 1, AH_K_FLUX0 = 2, AH_K_RECON1 = 3, AH_K_FLUX1 = 4,
