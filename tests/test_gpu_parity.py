@@ -750,6 +750,47 @@ def test_batch_equals_alone(use_graph, compact_every):
     _say(f"[gpu]   {nch} chunks, failed={sum(s['failed'] for s in st)}")
 
 
+def test_end_of_step_launch_and_fold_agree():
+    """A step ends in its last combination launch on plans with at most 64
+    combination tiles (256 cells each in 1D) per problem, and in a launch of
+    its own on the others.  A batch of the eleven small seeded Riemann
+    problems plus a 20 000-cell Sod problem (79 tiles: the plan takes the
+    separate launch) gives every small problem the state, step count and
+    failure status of running it alone (where the combination ends the step),
+    bit for bit; the large problem ends in the oracle's state after its 12
+    steps."""
+    from astrea_amd.plan import Plan
+    from astrea_amd import ic
+    import torch
+    grids, probs = _ensemble(12, 11, 32, 300)
+    n = 20000
+    w = ic.riemann_1d(n, (1., 0., 1.), (.125, 0., .1))
+    big = ic.initial_average(torch.from_numpy(w), 1.4, "edge", True).numpy()
+    grids.append(big)
+    probs.append(dict(n0=n, boundary="edge", dx=1 / n, gamma=1.4, cfl=.5, t_end=1e9, max_steps=12))
+    _say(f"[gpu] end-of-step launch vs fold: sizes={[p['n0'] for p in probs]}")
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13)
+    plan.load(grids)
+    plan.reset()
+    st = plan.run(chunk=8)
+    outs = plan.unpack()
+    plan.close()
+    for i in range(len(probs) - 1):
+        alone = Plan(1, "ppm", "hllc", "ssprk(3,3)", [probs[i]], varmask=0x13)
+        alone.load([grids[i]])
+        alone.reset()
+        s1 = alone.run(chunk=64)[0]
+        o1 = alone.unpack(0)
+        alone.close()
+        assert s1["steps"] == st[i]["steps"] and s1["failed"] == st[i]["failed"], (i, s1, st[i])
+        assert np.array_equal(o1, outs[i]), i
+    assert st[-1]["steps"] == 12 and not st[-1]["failed"]
+    s = O.Scheme(dimension=1, subgrid="ppm", solver="hllc", boundary="edge", gamma=1.4, cfl=.5, dx=1 / n)
+    ref = O.run(big, s, 1e9, 1, 12)
+    assert ref.steps == 12
+    assert np.max(rel_l1(outs[-1], ref.grid)) <= 1e-12
+
+
 def _totals_conserved(q0, q1):
     """Per-variable totals equal to 1e-13 of the total magnitude; each total
     is a pairwise sum over a contiguous copy (numpy sums strided reductions
