@@ -14,7 +14,7 @@ from time import perf_counter
 import torch
 
 from . import ic, options
-from .output import SnapshotFile
+from .output import open_snapshots
 from .plan import Plan, infer_varmask, solver_varmask
 
 
@@ -112,13 +112,15 @@ def save_run(path, grid, sim_variables, seed=None, every="step", precision=None)
     run's access key ('%Y%m%d%H%M%S' + microseconds) holding the run's
     snapshot series and the attributes config, cells, cfl, gamma, dimension,
     subgrid, timestep, solver and elapsed (seconds of core_run).  Appends to
-    an existing file.  Returns (access key, final grid, status)."""
+    an existing file.  A .hdf5 / .h5 path is the reference's HDF5 file
+    (written through libhdf5), any other name the zip archive of
+    ``output.SnapshotFile``.  Returns (access key, final grid, status)."""
     sv = sim_variables
     now = datetime.now()
     key = now.strftime('%Y%m%d%H%M%S') + str(now.microsecond)
     new = not os.path.exists(path)
     root = {"datetime": now.strftime('%Y%m%d%H%M'), "seed": seed} if new else None
-    with SnapshotFile(path, "w" if new else "a", attrs=root) as f:
+    with open_snapshots(path, "w" if new else "a", attrs=root) as f:
         lap = perf_counter()
         out, st = core_run(grid, sv, precision=precision, save=lambda k, a: f(f"{key}/{k}", a), every=every)
         elapsed = perf_counter() - lap

@@ -1,5 +1,9 @@
-"""The snapshot series sink (output.SnapshotFile) and the per-pass saving
-loop of core.run_saving, on the CPU with a scripted stand-in for the plan."""
+"""The snapshot series sinks (the reference's HDF5 file through libhdf5,
+output.Hdf5SnapshotFile, and the zip archive, output.SnapshotFile) and the
+per-pass saving loop of core.run_saving, on the CPU with a scripted stand-in
+for the plan."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -103,3 +107,70 @@ def test_snapshot_file_groups_and_attributes(tmp_path):
     with np.load(path) as z:
         assert np.array_equal(z["20261202120000123/0.5"], np.zeros((4, 8)))
         assert sum(k.endswith("/0.0") for k in z.files) == 2
+
+
+def _h5dump():
+    import shutil
+    return shutil.which("h5dump") or next((p for p in ("/opt/conda/bin/h5dump",) if os.path.exists(p)), None)
+
+
+def test_hdf5_snapshot_file_layout(tmp_path):
+    """The reference's HDF5 file (astrea.py:48-50, 151-153, 166-175, 186)
+    written through libhdf5: file attributes, a group per run with its
+    attributes (an attribute set again is replaced, as astrea.py:186 does for
+    'elapsed'), float64 snapshots named str(float(t)) with the attribute t,
+    appended runs; an existing snapshot name is refused, as h5py's
+    create_dataset refuses it."""
+    from astrea_amd import h5
+    from astrea_amd.output import Hdf5SnapshotFile, open_snapshots, read_attrs, read_snapshot, snapshot_keys
+    if not h5.available():
+        pytest.skip("libhdf5 cannot be loaded")
+    path = tmp_path / "runs.hdf5"
+    rng = np.random.default_rng(3)
+    a0, a1, a2 = rng.random((16, 8)), rng.random((300, 200, 8)), rng.random((5, 7, 8))
+    with open_snapshots(path, attrs={"datetime": "202612031200", "seed": 7}) as f:
+        assert isinstance(f, Hdf5SnapshotFile)
+        f("20261203120000123/0.0", a0)
+        f("20261203120000123/0.125", a1)
+        with pytest.raises(ValueError):
+            f("20261203120000123/0.125", a1)
+        f.set_attrs("20261203120000123", {"config": "sod", "cells": 16, "cfl": .5, "subgrid": "ppm"})
+        f.set_attrs("20261203120000123", {"elapsed": 1.5})
+        f.set_attrs("20261203120000123", {"elapsed": 2.5})
+        assert f.keys("20261203120000123") == ["0.0", "0.125"] and f.keys() == []
+    with open_snapshots(path, "a") as f:
+        f("20261203120001456/0.0", a2)
+        f.set_attrs("20261203120001456", {"config": "sod", "cells": 16, "seed": None})
+    assert read_attrs(path) == {"datetime": "202612031200", "seed": 7}
+    assert read_attrs(path, "20261203120000123") == {"config": "sod", "cells": 16, "cfl": .5, "subgrid": "ppm",
+                                                     "elapsed": 2.5}
+    assert read_attrs(path, "20261203120001456") == {"config": "sod", "cells": 16}
+    assert snapshot_keys(path, "20261203120000123") == ["0.0", "0.125"]
+    for k, a in (("20261203120000123/0.0", a0), ("20261203120000123/0.125", a1), ("20261203120001456/0.0", a2)):
+        assert np.array_equal(read_snapshot(path, k), a)
+    with h5.File(str(path), "r") as f:
+        assert f.read_attr("20261203120000123/0.125", "t") == 0.125
+        assert sorted(f.names()) == ["20261203120000123", "20261203120001456"]
+    # the format as the HDF5 tools see it: variable-length UTF-8 strings,
+    # 64-bit integers and floats, deflate level 9 on chunked datasets
+    tool = _h5dump()
+    if tool is None:
+        return
+    import subprocess
+    out = subprocess.run([tool, "-H", "-p", str(path)], capture_output=True, text=True, check=True).stdout
+    assert "STRSIZE H5T_VARIABLE;" in out and "CSET H5T_CSET_UTF8;" in out
+    assert 'ATTRIBUTE "seed" {\n      DATATYPE  H5T_STD_I64LE' in out
+    assert 'ATTRIBUTE "cfl" {\n         DATATYPE  H5T_IEEE_F64LE' in out
+    assert out.count("COMPRESSION DEFLATE { LEVEL 9 }") == 3
+    assert "DATASPACE  SIMPLE { ( 300, 200, 8 ) / ( 300, 200, 8 ) }" in out
+    assert out.count('ATTRIBUTE "t" {') == 3
+
+
+def test_zip_archive_for_other_names(tmp_path):
+    """Names other than .hdf5 / .h5 keep the zip archive of .npy members."""
+    from astrea_amd.output import open_snapshots, read_snapshot, snapshot_keys
+    path = tmp_path / "runs.npz"
+    with open_snapshots(path, attrs={"seed": 1}) as f:
+        assert isinstance(f, SnapshotFile)
+        f("g/0.5", np.ones((3, 8)))
+    assert snapshot_keys(path, "g") == ["0.5"] and np.array_equal(read_snapshot(path, "g/0.5"), np.ones((3, 8)))
