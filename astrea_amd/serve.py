@@ -166,14 +166,17 @@ class _Group:
 
 class Server:
     def __init__(self, hbm_budget=32 << 30, precision="fp64", compat="physical", cfl=.4, chunk=16,
-                 device="cuda:0", keep_results=False, bytes_per_cell=None, max_active_cells=None):
+                 device="cuda:0", keep_results=False, bytes_per_cell=None, max_active_cells=None, backfill=False):
         self.budget = hbm_budget
         # beyond the cells that fill the GPU, admitting more work adds no
         # throughput and only stretches the running requests (processor
         # sharing): an optional cap keeps the excess queued in arrival order,
         # trading median latency for tail latency.  A request larger than the
-        # cap still runs, alone.
+        # cap still runs, alone.  With backfill, requests smaller than a head
+        # of the queue that the cap holds back are admitted past it while
+        # they fit under the cap (the large ones keep their arrival order).
         self.max_active_cells = max_active_cells
+        self.backfill = backfill
         self.precision, self.compat, self.cfl, self.chunk = precision, compat, cfl, chunk
         self.device = device
         self.keep_results = keep_results
@@ -227,17 +230,27 @@ class Server:
                 waiting.append(pending.popleft())
             extra, admitted = {}, []
             active_cells = sum(x.n * x.n for g in self.groups.values() for x in g.reqs)
-            while waiting:
-                r = waiting[0]
+            held = None  # the head the cell cap holds back (backfill)
+            i = 0
+            while i < len(waiting):
+                r = waiting[i]
+                if held is not None and r.n >= held.n:
+                    i += 1
+                    continue
                 if (self.max_active_cells is not None and active_cells > 0
                         and active_cells + r.n * r.n > self.max_active_cells):
-                    break
+                    if not self.backfill:
+                        break
+                    held = held or r
+                    i += 1
+                    continue
                 trial = {k: list(v) for k, v in extra.items()}
                 trial.setdefault(r.subgrid, []).append(self.cost(r))
                 if self._replan_peak(trial) > self.budget:
                     break
                 extra = trial
-                admitted.append(waiting.popleft())
+                admitted.append(r)
+                del waiting[i]
                 active_cells += r.n * r.n
             idle = not admitted and not any(g.reqs or g.stale for g in self.groups.values())
             if waiting and idle and self._replan_peak({waiting[0].subgrid: [self.cost(waiting[0])]}) > self.budget:

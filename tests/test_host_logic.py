@@ -171,6 +171,39 @@ def test_active_cell_cap_bounds_the_batch(fake_gpu, monkeypatch):
         assert torch.all(r.result == r.status["steps"])
 
 
+def test_backfill_admits_small_requests_past_a_held_large_one(fake_gpu, monkeypatch):
+    """With backfill, a small request queued behind a large one that the cell
+    cap holds back is admitted at once; the large ones keep their arrival
+    order, and the cap still bounds every batch."""
+    spec = [(32, "plm", .5, 0.0), (32, "ppm", .5, 0.0), (8, "plm", .1, 0.0), (32, "plm", .3, 0.0),
+            (8, "ppm", .1, 0.0)]
+    cap = 32 * 32 + 2 * 8 * 8
+    seen = []
+    advance = serve._Group.advance
+
+    def watch(self, chunk, use_graph=True):
+        seen.append(sum(r.n * r.n for g in srv.groups.values() for r in g.reqs))
+        return advance(self, chunk, use_graph)
+
+    monkeypatch.setattr(serve._Group, "advance", watch)
+    order = {}
+    for backfill in (False, True):
+        seen.clear()
+        srv = serve.Server(hbm_budget=1 << 40, chunk=2, device="cpu", keep_results=True, bytes_per_cell=1,
+                           max_active_cells=cap, backfill=backfill)
+        out = srv.run(_requests(spec), realtime=False)
+        assert out["requests"] == len(spec) and max(seen) <= cap
+        for r in out["done"]:
+            assert torch.all(r.result == r.status["steps"])
+        order[backfill] = {r.rid: r.admitted_at for r in out["done"]}
+    plain, bf = order[False], order[True]
+    # without backfill the small requests wait for the large one ahead of them
+    assert plain[2] > plain[1] and plain[4] > plain[1]
+    # with it they start beside the first request, before the held one
+    assert bf[2] < bf[1] and bf[4] < bf[1]
+    assert bf[1] < bf[3]
+
+
 def test_requests_without_grids_start_on_the_device(fake_gpu):
     reqs = _requests(SPEC)
     for r in reqs:
