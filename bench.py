@@ -373,6 +373,40 @@ def serve_leg(args):
     }
 
 
+def ensemble_bulk_roofline(grids, probs, blocks, tile_s, reps=3):
+    """The 1D sweep launch of the ensemble's first step, in which every
+    problem is active in every launch (a problem that fails in it still runs
+    the step's three sweeps): per-launch time from HIP events on the plan's
+    stream (the step repeated from the initial state, the last repetition
+    kept), against the algorithmic bytes of the three sweeps of an SSPRK(3,3)
+    step, 3 FP64 components per cell -- stage 0 reads U and writes L0
+    (48 B/cell), the middle stage reads U and L0 and writes K2 (72 B/cell),
+    the last reads K2 and writes L (48 B/cell): 56 B/cell per launch on
+    average."""
+    from astrea_amd import workloads as W
+    from astrea_amd.plan import Plan
+    plan = Plan(1, "ppm", "hllc", "ssprk(3,3)", probs, varmask=0x13, tile_s=tile_s)
+    plan.set_blocks(blocks)
+    for _ in range(reps):
+        W._start(plan, grids)
+        plan.reset()
+        plan.profile(16)
+        plan.step(1)
+        prof = plan.profile_read()
+    plan.close()
+    cells = sum(p["n0"] for p in probs)
+    ms, n = prof["sweep1d"]
+    per_launch = ms / n
+    nbytes = 56.0 * cells
+    achieved = nbytes / (per_launch * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "sweep1d",
+            "kernel_avg_ms": per_launch, "bytes_per_launch": nbytes,
+            "combine_avg_ms": prof["combine"][0] / prof["combine"][1] if "combine" in prof else None,
+            "phase": f"the first step, all {cells} cells active; the tail of a few large problems that sets the "
+                     "run's length is bound by launch latency, not by this figure"}
+
+
 def bench_ensemble(args):
     """Ragged 1D Riemann ensemble (512 problems), batched in one plan
     (persistent tile loops, hipGraph-captured chunks, compaction) against the
@@ -404,6 +438,8 @@ def bench_ensemble(args):
             continue
         o, n = ref["offsets"][i], probs[i]["n0"]
         errs.append(W.rel_l1(res["split"]["state"][:, o:o + n], ref["state"][:, o:o + n]))
+    roof = ensemble_bulk_roofline(grids, probs, blocks, tile_s)
+    _progress(f"[ensemble] bulk sweep launch {roof['kernel_avg_ms'] * 1e3:.1f} us, {roof['achieved']:.0f} GB/s")
     rng = np.random.default_rng(SEED)
     sample = sorted(rng.choice(len(probs), min(args.plain_sample, len(probs)), replace=False).tolist())
     plain = W.run_ensemble_plain(grids, probs, sample=sample)
@@ -425,6 +461,7 @@ def bench_ensemble(args):
         "reference_literal_hllc_fp64": None if literal is None else
         {"zu_per_s": literal["zu_per_s"], "failed": sum(s["failed"] for s in literal["status"])},
         "speedup_vs_plain": value / plain["zu_per_s"],
+        "roofline": roof,
         "split_l1_vs_fp64": {"max": max(errs) if errs else None, "median": float(np.median(errs)) if errs else None,
                              "p90": float(np.percentile(errs, 90)) if errs else None, "compared": len(errs),
                              "failure_mismatch": mism},
