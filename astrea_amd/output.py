@@ -94,7 +94,11 @@ class Hdf5SnapshotFile:
         self.path = path
         self._f = h5.File(path, mode)
         if attrs is not None:
-            self.set_attrs("", attrs)
+            try:
+                self.set_attrs("", attrs)
+            except Exception:
+                self.close()
+                raise
 
     def __call__(self, key, snapshot):
         key = str(key)
