@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--bytes", type=int, default=1 << 30, help="copy: bytes per copy")
     ap.add_argument("--nprob", type=int, default=512)
     ap.add_argument("--plain-sample", type=int, default=24)
+    ap.add_argument("--ens-bulk-only", action="store_true",
+                    help="ensemble: only the first-step sweep measurement (tiles of 243 cells, 4096 workgroups: the "
+                         "autotuner's pick), for counter runs")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--rate", type=float, default=3.0,
                     help="serve: arrivals per second (3.0/s loads the plain FP64 loop to ~83%%: the stream "
@@ -399,9 +402,22 @@ def ensemble_bulk_roofline(grids, probs, blocks, tile_s, reps=3):
     per_launch = ms / n
     nbytes = 56.0 * cells
     achieved = nbytes / (per_launch * 1e-3) / 1e9
+    # HBM bytes and VALU instructions of that launch from the counter runs
+    # summarised in profiles/ (this 512-problem ensemble only)
+    traffic, valu, tsrc = None, None, None
+    tpath = os.path.join(ROOT, "profiles", "traffic_sweep1d_ensemble512.json")
+    if len(probs) == 512 and os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic, tsrc = tj.get("bytes_per_launch"), tj.get("source")
+        insts = tj.get("valu_wave_insts_per_launch")
+        if insts:
+            issue_s = insts * 4 / (1024 * 2.4e9)
+            valu = {"wave_insts_per_launch": insts, "issue_ms": issue_s * 1e3,
+                    "frac": issue_s / (per_launch / 1e3), "source": tsrc}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "sweep1d",
-            "kernel_avg_ms": per_launch, "bytes_per_launch": nbytes,
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic, "traffic_source": tsrc, "valu": valu,
+            "kernel": "sweep1d", "kernel_avg_ms": per_launch, "bytes_per_launch": nbytes,
             "combine_avg_ms": prof["combine"][0] / prof["combine"][1] if "combine" in prof else None,
             "phase": f"the first step, all {cells} cells active; the tail of a few large problems that sets the "
                      "run's length is bound by launch latency, not by this figure"}
@@ -415,6 +431,13 @@ def bench_ensemble(args):
     import torch
     from astrea_amd import workloads as W
     grids, probs = W.ensemble_inputs(args.nprob, SEED)
+    if args.ens_bulk_only:
+        roof = ensemble_bulk_roofline(grids, probs, 4096, 0)
+        return {"metric": "first-step 1D sweep launch of the ensemble", "value": roof["achieved"], "unit": "GB/s",
+                "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                "config": {"workload": f"{args.nprob} ragged 1D Riemann problems, first step", "seed": SEED},
+                "roofline": roof}
     (tile_s, blocks), tune = W.autotune(grids, probs, "fp64")
     res = {}
     paths = args.ens_paths.split(",")
