@@ -1011,6 +1011,63 @@ def test_quadrants_physical_compat():
         assert np.max(rel_l1(out, ref.grid)) <= 1e-12
 
 
+@pytest.mark.parametrize("seed", range(10))
+def test_random_rectangular_quadrants_vs_oracle(seed):
+    """Seeded random cases beyond the configuration table: rectangular grids
+    whose sides are no multiple of any tile size, four random quadrant states
+    (density 0.1-2, velocities -1..1, pressure 0.05-2; seeds 8 and 9 with
+    contrasts up to 100), the subgrid, solver, boundary, compatibility mode
+    and flattener cycling with the seed, 8 steps.  The device ends in the
+    oracle's state (rel L1 <= 1e-12; 5e-11 with WENO) after the same steps,
+    or fails where the oracle with the reference's LAPACK eigenvalues raises
+    and keeps the oracle's last state."""
+    from astrea_amd.plan import Plan
+    rng = np.random.default_rng(7100 + seed)
+    n0, n1 = (int(x) for x in rng.integers(17, 42, size=2))
+    sg = ["pcm", "plm", "ppm", "weno5", "ppm"][seed % 5]
+    solver = ["lf", "hllc"][seed % 2]
+    bc = ["edge", "wrap"][(seed // 2) % 2]
+    compat = ["astrea", "physical"][(seed // 4) % 2]
+    flat = ["off", "astrea", "colella90"][seed % 3] if sg == "ppm" else "off"
+    x0, y0 = rng.uniform(.3, .7, size=2)
+    x = (np.arange(n0) + .5) / n0
+    y = (np.arange(n1) + .5) / n1
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    quad = (X >= x0).astype(int) + 2 * (Y >= y0).astype(int)
+    lo = .01 if seed >= 8 else .1
+    states = np.zeros((4, 8))
+    states[:, 0] = rng.uniform(lo, 1, 4) * (10 if seed >= 8 else 2)
+    states[:, 1:3] = rng.uniform(-1, 1, (4, 2))
+    states[:, 4] = rng.uniform(lo / 2, 1, 4) * (10 if seed >= 8 else 2)
+    w = states[quad]
+    dx = 1 / n0
+    s = O.Scheme(dimension=2, subgrid=sg, solver=solver, boundary=bc, gamma=1.4, cfl=.4, dx=dx, compat=compat,
+                 flattener=flat)
+    q0 = O.initial_average(w, s)
+    _say(f"[gpu] random quadrants seed={seed}: {n0}x{n1} {sg}/{solver}/{bc}/{compat}/flattener={flat}")
+    prob = dict(n0=n0, n1=n1, boundary=bc, dx=dx, gamma=1.4, cfl=.4, t_end=1e9, max_steps=8)
+    plan = Plan(2, sg, solver, "ssprk(3,3)", [prob], varmask=0x17, compat=compat, flattener=flat)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(16)[0]
+    out = plan.unpack(0)
+    plan.close()
+    tol = 5e-11 if sg == "weno5" else 1e-12
+    with np.errstate(all="ignore"):
+        if st["failed"]:
+            n_ok = st["steps"]
+            _say(f"[gpu]   failed after {n_ok} steps")
+            with pytest.raises(np.linalg.LinAlgError):
+                O.run(q0, replace(s, eig="lapack"), 1e9, 1, n_ok + 1)
+            ref = O.run(q0, s, 1e9, 1, n_ok)
+        else:
+            ref = O.run(q0, s, 1e9, 1, 8)
+            assert st["steps"] == ref.steps == 8
+    e = float(np.max(rel_l1(out, ref.grid)))
+    _say(f"[gpu]   {st['steps']} steps, rel L1 vs oracle {e:.1e}")
+    assert e <= tol
+
+
 MHD = sorted(k for k, v in meta()["cases"].items() if v.get("magnetic_2d"))
 
 
