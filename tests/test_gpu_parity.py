@@ -1068,6 +1068,58 @@ def test_random_rectangular_quadrants_vs_oracle(seed):
     assert e <= tol
 
 
+RK_ALL = ["euler", "ssprk(2,2)", "ssprk(3,3)", "ssprk(4,3)", "rk4", "ssprk(5,3)", "ssprk(5,4)", "ssprk(10,4)"]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_1d_riemann_vs_oracle(seed):
+    """Seeded random 1D Riemann problems off the configuration table: the
+    time stepper cycles through all eight schemes, the subgrid through
+    PCM/PLM/PPM/WENO-3/5/7, with LLF or HLLC, either boundary and
+    compatibility mode, 100-700 cells (several tiles), random left/right
+    states, 10 steps: the oracle's state after the same steps (rel L1
+    <= 1e-12; 5e-11 with WENO-7), or a failure where the oracle with the
+    reference's LAPACK eigenvalues raises, keeping the oracle's last state."""
+    from astrea_amd.plan import Plan
+    rng = np.random.default_rng(9100 + seed)
+    n = int(rng.integers(100, 701))
+    ts = RK_ALL[seed % 8]
+    sg = ["pcm", "plm", "ppm", "weno3", "weno5", "weno7"][seed % 6]
+    solver = ["lf", "hllc"][(seed // 2) % 2]
+    bc = ["edge", "wrap"][(seed // 3) % 2]
+    compat = ["astrea", "physical"][(seed // 5) % 2]
+    left, right = np.zeros(8), np.zeros(8)
+    for st_ in (left, right):
+        st_[0], st_[1], st_[4] = rng.uniform(.1, 2), rng.uniform(-1, 1), rng.uniform(.05, 2)
+    x0 = rng.uniform(.3, .7)
+    w = np.where(((np.arange(n) + .5) / n < x0)[:, None], left, right)
+    s = O.Scheme(dimension=1, subgrid=sg, solver=solver, timestep=ts, boundary=bc, gamma=1.4, cfl=.5, dx=1 / n,
+                 compat=compat)
+    q0 = O.initial_average(w, s)
+    _say(f"[gpu] random 1D seed={seed}: n={n} {sg}/{solver}/{ts}/{bc}/{compat}")
+    prob = dict(n0=n, boundary=bc, dx=1 / n, gamma=1.4, cfl=.5, t_end=1e9, max_steps=10)
+    plan = Plan(1, sg, solver, ts, [prob], varmask=0x13, compat=compat)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(16)[0]
+    out = plan.unpack(0)
+    plan.close()
+    tol = 5e-11 if sg == "weno7" else 1e-12
+    with np.errstate(all="ignore"):
+        if st["failed"]:
+            n_ok = st["steps"]
+            _say(f"[gpu]   failed after {n_ok} steps")
+            with pytest.raises(np.linalg.LinAlgError):
+                O.run(q0, replace(s, eig="lapack"), 1e9, 1, n_ok + 1)
+            ref = O.run(q0, s, 1e9, 1, n_ok)
+        else:
+            ref = O.run(q0, s, 1e9, 1, 10)
+            assert st["steps"] == ref.steps == 10
+    e = float(np.max(rel_l1(out, ref.grid)))
+    _say(f"[gpu]   {st['steps']} steps, rel L1 vs oracle {e:.1e}")
+    assert e <= tol
+
+
 MHD = sorted(k for k, v in meta()["cases"].items() if v.get("magnetic_2d"))
 
 
