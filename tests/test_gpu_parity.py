@@ -1068,6 +1068,61 @@ def test_random_rectangular_quadrants_vs_oracle(seed):
     assert e <= tol
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_random_mhd_2d_vs_oracle(seed):
+    """Seeded random 2D MHD fields through the constrained-transport path:
+    rectangular grids (12-24 cells a side), Orszag-Tang-like sine fields with
+    random amplitudes, phases and a random uniform background, the subgrid
+    (PLM/PPM/WENO-5), solver (LLF/HLLC/HLLD), boundary and time stepper
+    (SSPRK(3,3)/RK4/SSPRK(2,2)) cycling with the seed, 4 steps: the oracle's
+    state (rel L1 <= 1e-12), or a failure where the oracle with the
+    reference's LAPACK eigenvalues raises, keeping the oracle's last state."""
+    from astrea_amd.plan import Plan
+    rng = np.random.default_rng(9700 + seed)
+    n0, n1 = (int(x) for x in rng.integers(12, 25, size=2))
+    sg = ["plm", "ppm", "weno5"][seed % 3]
+    solver = ["lf", "hlld", "hllc"][(seed // 2) % 3]
+    bc = ["wrap", "edge"][seed % 2]
+    ts = ["ssprk(3,3)", "rk4", "ssprk(2,2)"][(seed // 3) % 3]
+    x = (np.arange(n0) + .5) / n0
+    y = (np.arange(n1) + .5) / n1
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    ph = rng.uniform(0, 1, 4)
+    a = rng.uniform(.5, 1.5, 4)
+    w = np.zeros((n0, n1, 8))
+    w[..., 0] = 25 / (36 * np.pi) * (1 + .3 * a[0] * np.sin(2 * np.pi * (X + ph[0])))
+    w[..., 4] = 5 / (12 * np.pi) * (1 + .2 * a[1] * np.cos(2 * np.pi * (Y + ph[1])))
+    w[..., 1] = -a[2] * np.sin(2 * np.pi * (Y + ph[2])) + rng.uniform(-.3, .3)
+    w[..., 2] = a[3] * np.sin(2 * np.pi * (X + ph[3])) + rng.uniform(-.3, .3)
+    w[..., 5] = -np.sin(2 * np.pi * (Y + ph[0])) / np.sqrt(4 * np.pi) + rng.uniform(-.1, .1)
+    w[..., 6] = np.sin(4 * np.pi * (X + ph[1])) / np.sqrt(4 * np.pi) + rng.uniform(-.1, .1)
+    dx = 1 / n0
+    s = O.Scheme(dimension=2, subgrid=sg, solver=solver, timestep=ts, boundary=bc, gamma=5 / 3, cfl=.4, dx=dx,
+                 magnetic_2d=True)
+    q0 = O.initial_average(w, s)
+    _say(f"[gpu] random mhd seed={seed}: {n0}x{n1} {sg}/{solver}/{ts}/{bc}")
+    prob = dict(n0=n0, n1=n1, boundary=bc, dx=dx, gamma=5 / 3, cfl=.4, t_end=1e9, max_steps=4)
+    plan = Plan(2, sg, solver, ts, [prob], varmask=0x77, magnetic_2d=True)
+    plan.load([q0])
+    plan.reset()
+    st = plan.run(8)[0]
+    out = plan.unpack(0)
+    plan.close()
+    with np.errstate(all="ignore"):
+        if st["failed"]:
+            n_ok = st["steps"]
+            _say(f"[gpu]   failed after {n_ok} steps")
+            with pytest.raises(np.linalg.LinAlgError):
+                O.run(q0, replace(s, eig="lapack"), 1e9, 1, n_ok + 1)
+            ref = O.run(q0, s, 1e9, 1, n_ok)
+        else:
+            ref = O.run(q0, s, 1e9, 1, 4)
+            assert st["steps"] == ref.steps == 4
+    e = float(np.max(rel_l1(out, ref.grid)))
+    _say(f"[gpu]   {st['steps']} steps, rel L1 vs oracle {e:.1e}")
+    assert e <= 1e-12
+
+
 RK_ALL = ["euler", "ssprk(2,2)", "ssprk(3,3)", "ssprk(4,3)", "rk4", "ssprk(5,3)", "ssprk(5,4)", "ssprk(10,4)"]
 
 
