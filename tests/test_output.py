@@ -174,3 +174,25 @@ def test_zip_archive_for_other_names(tmp_path):
         assert isinstance(f, SnapshotFile)
         f("g/0.5", np.ones((3, 8)))
     assert snapshot_keys(path, "g") == ["0.5"] and np.array_equal(read_snapshot(path, "g/0.5"), np.ones((3, 8)))
+
+
+def test_hdf5_chunks_and_refused_values(tmp_path):
+    """Chunks of an auto-chunked snapshot hold at most 1 MiB and tile the
+    array's shape; attribute values the reference never writes (bool, None
+    as a single value, containers) are refused rather than stored in some
+    other HDF5 type."""
+    from astrea_amd import h5
+    for shape in ((1024, 1024, 8), (64, 8), (3, 5, 7), (40000, 8)):
+        c = h5.guess_chunk(shape, 8)
+        assert len(c) == len(shape) and all(1 <= a <= b for a, b in zip(c, shape))
+        assert int(np.prod(c)) * 8 <= h5.CHUNK_MAX or int(np.prod(c)) == 1
+    if not h5.available():
+        pytest.skip("libhdf5 cannot be loaded")
+    with h5.File(str(tmp_path / "a.hdf5"), "w") as f:
+        f.require_group("g")
+        for bad in (True, None, [1, 2], {"a": 1}):
+            with pytest.raises(TypeError):
+                f.set_attr("g", "x", bad)
+        f.set_attr("g", "n", np.int64(3))
+        f.set_attr("g", "v", np.float32(.5))
+        assert f.read_attr("g", "n") == 3 and f.read_attr("g", "v") == .5
